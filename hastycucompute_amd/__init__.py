@@ -1,0 +1,15 @@
+"""MI355X (gfx950) non-Cartesian reconstruction hot path.
+
+Kaiser-Bessel NUFFT (type-1 spread, type-2 interp), Toeplitz-embedded SENSE
+normal operator, and FP64-scalar CG, as hand-written HIP kernels behind the
+C-ABI in ``include/hasty_hip.h`` (``libhasty_hip.so`` in this directory).
+The Python modules mirror the reference's corelib operator API:
+
+* ``hastycucompute_amd.fft`` -- ``nufft_plan``, ``toeplitz_kernel``,
+  ``toeplitz_multiply``, ``toeplitz_multiplication`` ...
+* ``hastycucompute_amd.mri`` -- ``NORMAL_T_T1_OP``, ``SenseNufft``
+* ``hastycucompute_amd.op``  -- ``conjugate_gradient``, ``max_eig``
+"""
+from ._lib import LIB_PATH, HastyError, lib  # noqa: F401
+
+__all__ = ["LIB_PATH", "HastyError", "lib"]

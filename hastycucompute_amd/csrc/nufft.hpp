@@ -1,0 +1,105 @@
+// NUFFT plan (type 1 spread / type 2 interp) for gfx950.
+//
+// Replaces nufft_plan<cuda_t,TT,DIM,NT> (nufft.cppm:103-310), which wraps
+// cufinufft.  Internally every transform is 3-D with slots (z, y, x); a 2-D
+// transform has n_z = nf_z = 1.
+//
+// Data layout on the device (per internal batch of transforms):
+//   fine  [B, nf_z, nf_y, nf_x]   oversampled grid (x fastest)
+//   bufB  [B, n_z,  nf_y, nf_x]   after the z-axis pass (pruned)
+//   bufA  [B, n_z,  n_y,  nf_x]   after the y-axis pass (pruned)
+// Points are kept bin-sorted by fine-grid tile (T_z x T_y x T_x cells):
+//   sidx[s]          original index of sorted point s
+//   sl0[a][s]        first support cell along slot a (wrapped into [0, nf_a))
+//   sfrac[a][s]      u_a - l0_a in (w/2 - 1, w/2]  (fractional offset, R)
+//   visits[v]        8*s + (tile-neighbour bits) for spreading: tile-major
+//                    list of the points whose support touches each tile.
+#pragma once
+
+#include <vector>
+
+#include "common.hpp"
+#include "fft.hpp"
+
+namespace hasty {
+
+constexpr int kNufftThreads = 256;
+constexpr int kMaxGroup = 4;        // transforms per spread / interp block
+
+struct NufftPlan {
+    int type = 1, dim = 3, sign = 1, ntransf = 1;
+    bool f64 = false;
+    double tol = 1e-5, sigma = 2.0, beta = 0;
+    int w = 6;
+    int64_t n[3] = {1, 1, 1};
+    int64_t nf[3] = {1, 1, 1};
+    int T[3] = {1, 1, 1};
+    int ntile[3] = {1, 1, 1};
+    int64_t ntiles = 1;
+    int group = 1;
+    int max_batch = 1;
+    int max_sub = 4096;
+    hipStream_t stream = nullptr;
+    int device = 0;
+
+    void* deapod[3] = {nullptr, nullptr, nullptr};
+    void* fine = nullptr;
+    void* bufA = nullptr;
+    void* bufB = nullptr;
+
+    int64_t M = 0;
+    bool have_pts = false;
+    bool gm = false;             // global-memory spread / interp kernels
+    int* sidx = nullptr;
+    int* sl0 = nullptr;          // [3][M]
+    void* sfrac = nullptr;       // [3][M] of R
+    int* visits = nullptr;
+    int64_t nvisits = 0;
+    int4* sub_interp = nullptr;
+    int nsub_interp = 0;
+    int4* sub_spread = nullptr;
+    int nsub_spread = 0;
+    int* split_tiles = nullptr;
+    int nsplit = 0;
+    int64_t pts_cap = 0, visits_cap = 0;
+    int64_t sub_interp_cap = 0, sub_spread_cap = 0, split_cap = 0;
+
+    int64_t device_bytes = 0;
+
+    ~NufftPlan();
+};
+
+NufftPlan* nufft_makeplan(int type, int dim, const int64_t* nmodes, int sign, int ntransf,
+                          double tol, bool f64, const hasty_nufft_opts* opts, hipStream_t stream);
+void nufft_setpts(NufftPlan* p, int64_t M, const void* c0, const void* c1, const void* c2);
+void nufft_execute(NufftPlan* p, void* c, void* f);
+
+// Pruned multi-dimensional FFT helpers shared with the Toeplitz code.
+// Geometry in slots (z, y, x): image extent n, FFT length L, shifts.
+struct PadGeometry {
+    int64_t n[3];
+    int64_t L[3];
+    int shift_in[3];    // image index j -> line position (j + shift) mod L
+    int shift_out[3];   // output index j <- line position (j + shift) mod L
+};
+
+// img [B, n] (row loads with optional diagonals and deapodisation) ->
+// grid [B, L] = FFT_sign(pad(img)), using bufA [B, n_z, n_y, L_x] and
+// bufB [B, n_z, L_y, L_x].
+template <class V>
+void pruned_forward(const PadGeometry& g, int64_t B, const V* img, V* grid, V* bufA, V* bufB,
+                    int sign, const DiagOp<V>& m1, const DiagOp<V>& m2,
+                    const decltype(V::x)* const* deapod, hipStream_t s);
+
+// grid [B, L] -> img [B, n] = crop(FFT_sign(grid)) (.) diagonals, deapod,
+// scale, optional accumulate / sum over B.
+template <class V>
+void pruned_inverse(const PadGeometry& g, int64_t B, const V* grid, V* img, V* bufA, V* bufB,
+                    int sign, decltype(V::x) scale, const DiagOp<V>& m1, const DiagOp<V>& m2,
+                    const DiagOp<V>& io, const decltype(V::x)* const* deapod, int accumulate,
+                    int sum_batch, hipStream_t s);
+
+template <class V> ColArgs<V> make_col_args(int L);
+template <class V> int rows_per_block(int L, int out_len);
+
+}  // namespace hasty

@@ -1,0 +1,160 @@
+/*
+ * hasty_hip.h -- C-ABI of the MI355X (gfx950) non-Cartesian reconstruction path.
+ *
+ * Plain pointers and sizes only.  Every device pointer is caller-owned memory
+ * on the current HIP device; every call enqueues work on the given stream
+ * (NULL = the default stream) and returns without a device-wide sync unless
+ * stated.  Status 0 = OK; on failure hasty_last_error() returns a message
+ * (thread-local).
+ *
+ * Layouts (C-order, last axis fastest), complex data interleaved (re, im):
+ *   image / modes   [ntransf, n0, n1, n2]     (n0 slowest)
+ *   nonuniform data [ntransf, M]
+ *   coordinates     dim separate real arrays [M], radians, coords[a] <-> axis a
+ *   Toeplitz kernel [2 n0, 2 n1, 2 n2]
+ *   coil maps       [C, n0, n1, n2]
+ * Complex dtypes: c64 = 2 x float, c128 = 2 x double.
+ */
+#ifndef HASTY_HIP_H
+#define HASTY_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define HASTY_OK 0
+#define HASTY_ERR_INVALID 1   /* bad argument / shape / enum            */
+#define HASTY_ERR_ALLOC 2     /* device or host allocation failed       */
+#define HASTY_ERR_HIP 3       /* a HIP runtime call failed              */
+#define HASTY_ERR_STATE 4     /* plan misuse (execute before setpts ...) */
+#define HASTY_ERR_INTERNAL 5
+
+/* ---- enums mirrored from the reference --------------------------------- */
+/* nufft_type (nufft.cppm:42-48) */
+#define HASTY_TYPE_1 1        /* BACKWARD: nonuniform -> modes (spread)  */
+#define HASTY_TYPE_2 2        /* FORWARD:  modes -> nonuniform (interp)  */
+/* nufft_method_cuda (nufft.cppm:65-70) is accepted for API parity; this
+ * library always uses its bin-sorted LDS-tiled method.                    */
+/* ToeplitzMultType / ToeplitzAccumulateType (toeplitz.cppm:47-56) */
+#define HASTY_MULT_NONE 0
+#define HASTY_MULT 1
+#define HASTY_MULT_CONJ 2
+#define HASTY_ACCUMULATE_NONE 0
+#define HASTY_ACCUMULATE 1
+
+/* precision of a plan: real type of coordinates and data */
+#define HASTY_F32 0           /* coords float,  data c64  */
+#define HASTY_F64 1           /* coords double, data c128 */
+
+typedef struct hasty_nufft_plan_s* hasty_nufft_plan;
+typedef struct hasty_normal_op_s* hasty_normal_op;
+
+/* Optional plan tuning (NULL = defaults).  Mirrors the option block of
+ * nufft_opts (nufft.cppm:78-95) plus this library's tiling knobs. */
+typedef struct hasty_nufft_opts {
+    double upsamp;        /* oversampling sigma; <= 0 -> 2.0 (nufft_upsamp_cuda)  */
+    int32_t method;       /* accepted, ignored (nufft_method_cuda)                */
+    int32_t tile[3];      /* fine-grid tile per axis (slots z,y,x); 0 -> auto     */
+    int32_t coil_group;   /* transforms per spread/interp block; 0 -> auto        */
+    int32_t max_batch;    /* transforms per internal batch; 0 -> all              */
+    int32_t max_subproblem; /* points per spread/interp block; 0 -> auto          */
+} hasty_nufft_opts;
+
+const char* hasty_last_error(void);
+int hasty_version(void);
+int hasty_device_synchronize(void);
+/* Launch timing for the calling thread: when enabled, every kernel launch of
+ * the library is bracketed by a HIP event pair on its stream.  read()
+ * synchronises on the recorded events, aggregates elapsed milliseconds and
+ * launch counts per kernel name ('\n'-separated in names), and clears. */
+int hasty_profile_enable(int on);
+int hasty_profile_read(char* names, int64_t names_len, double* ms, int32_t* counts,
+                       int32_t max_entries, int32_t* n_entries);
+
+/* ---- NUFFT plan: replaces nufft_plan<cuda_t,TT,DIM,NT> (nufft.cppm:103-310),
+ *      the cufinufft(f)_makeplan / setpts / execute / destroy call sites
+ *      (nufft.cppm:160-164, 191-198, 214-245, 282-285). ------------------- */
+int hasty_nufft_makeplan(int type, int dim, const int64_t* nmodes, int sign, int ntransf,
+                         double tol, int precision, const hasty_nufft_opts* opts,
+                         void* stream, hasty_nufft_plan* plan_out);
+/* coordinates are copied into the plan's bin-sorted layout; the caller's
+ * arrays may be freed after the call returns and the stream has run it.
+ * c0 pairs with axis 0 (slowest), c_{dim-1} with the fastest axis; unused
+ * pointers may be NULL. M == 0 is allowed (execute then writes zeros / nothing). */
+int hasty_nufft_setpts(hasty_nufft_plan plan, int64_t M, const void* c0, const void* c1,
+                       const void* c2);
+/* FINUFFT argument convention: c = nonuniform data [ntransf, M],
+ * f = modes [ntransf, n...].  Type 1 reads c and writes f; type 2 reads f
+ * and writes c. */
+int hasty_nufft_execute(hasty_nufft_plan plan, void* c, void* f);
+int hasty_nufft_set_stream(hasty_nufft_plan plan, void* stream);
+int hasty_nufft_destroy(hasty_nufft_plan plan);
+/* introspection: width, beta, fine grid, tiles, device bytes held */
+int hasty_nufft_info(hasty_nufft_plan plan, int* width, double* beta, int64_t* nf3,
+                     int32_t* tile3, int64_t* device_bytes);
+
+/* ---- Toeplitz ----------------------------------------------------------- */
+/* toeplitz_kernel (nufft.cppm:386-441): kernel[2n...] = fftn(embed(psf)) / prod(n),
+ * psf[m] = sum_j w_j e^{+i k_j.m}, m in [-(n-1), n-1], lag m stored at index
+ * m + n - 1 (index 2n-1 stays zero).  weights = c64 [M] or NULL (all ones).
+ * The PSF is computed with an FP64 type-1 NUFFT at tolerance tol (<= 0 ->
+ * 1e-9) and an FP64 FFT, then rounded to c64. coords are float [M]. */
+int hasty_toeplitz_kernel(int dim, const int64_t* n, int64_t M, const void* c0,
+                          const void* c1, const void* c2, const void* weights, void* kernel,
+                          double tol, void* stream);
+/* toeplitz_multiply (nufft.cppm:448-479): inout[b] <- crop_{n-1}(ifftn(fftn_{2n}(inout[b]) * kernel)),
+ * batch >= 1, c64. */
+int hasty_toeplitz_multiply(int dim, const int64_t* n, int64_t batch, void* inout,
+                            const void* kernel, void* stream);
+/* transform_toeplitz_kernel (toeplitz.cppm:510-609): the kernel is kept in
+ * natural frequency order here, so this is the identity (validates shape). */
+int hasty_transform_toeplitz_kernel(int dim, const int64_t* n, void* kernel, void* stream);
+/* toeplitz_multiplication (toeplitz.cppm:611-712, load/store rules of
+ * toeplitz_load_3D.cu:18-79): for each b in [0,batch)
+ *   s      = input[b] (.)in_m1 mult1 (.)in_m2 mult2
+ *   out[b] = crop(T s) (.)out_m1 mult1 (.)out_m2 mult2 (.)io input[b]  (+ out[b])
+ * types[5] = {input_output, input_mult1, output_mult1, input_mult2, output_mult2}
+ * mult1 / mult2 may be NULL (then their types are ignored); scratch may be
+ * NULL (the library allocates stream-ordered) or caller memory of
+ * scratch_bytes >= hasty_toeplitz_scratch_bytes(dim, n, batch). */
+int hasty_toeplitz_multiplication(int dim, const int64_t* n, int64_t batch, const void* input,
+                                  void* output, const void* kernel, void* scratch,
+                                  int64_t scratch_bytes, const void* mult1, const void* mult2,
+                                  const int32_t* types, int accumulate, void* stream);
+int64_t hasty_toeplitz_scratch_bytes(int dim, const int64_t* n, int64_t batch);
+
+/* ---- SENSE normal operator: replaces NORMAL_T_T1_OP (normal.cppm:43-93):
+ *      out = sum_c conj(S_c) . T(S_c . x), all coils batched per launch,
+ *      coil sum fused into the last pass (no [C, n...] intermediate). ------ */
+int hasty_normal_create(int dim, const int64_t* n, int ncoils, const void* smaps,
+                        const void* kernel, int coil_batch, void* stream, hasty_normal_op* op);
+int hasty_normal_apply(hasty_normal_op op, const void* x, void* out);
+int hasty_normal_set_stream(hasty_normal_op op, void* stream);
+int hasty_normal_destroy(hasty_normal_op op);
+
+/* ---- CG / vector ops (cg.cppm:122-184, alg.cppm:13-30) with FP64
+ *      accumulation.  state = caller device buffer of HASTY_CG_STATE_DOUBLES
+ *      doubles; all scalars stay on the device. ------------------------------ */
+#define HASTY_CG_STATE_DOUBLES 1040
+/* out[0] = Re sum conj(a) b, out[1] = Im (device doubles), n complex elements */
+int hasty_dot(const void* a, const void* b, int64_t n, double* out, void* stream);
+/* state <- {rz = ||r||^2, done = sqrt(rz) < tol}; p <- r */
+int hasty_cg_init(void* r, void* p, int64_t n, double tol, double* state, void* stream);
+/* one CG update given Ap: pAp, alpha, x += alpha p, r -= alpha Ap, rz', p = r + (rz'/rz) p.
+ * No-op once state.done is set.  */
+int hasty_cg_step(void* x, void* r, void* p, const void* Ap, int64_t n, double* state,
+                  void* stream);
+/* host read of residual and convergence flag (synchronises the stream) */
+int hasty_cg_status(const double* state, double* resid, int* done, void* stream);
+/* y <- a * x (a real FP64 scalar from device memory, scaled by 1/a_ptr[0] if invert) */
+int hasty_scale_by_device(void* x, int64_t n, const double* a, int invert, void* stream);
+/* out[0] = ||x||_2 (FP64, device) */
+int hasty_norm(const void* x, int64_t n, double* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HASTY_HIP_H */

@@ -1,0 +1,185 @@
+"""GPU parity: NUFFT type 1 / type 2 through the C-ABI vs the FP64 oracle.
+
+Bar (from the path's accuracy contract): relative L2 <= 2e-5 against the FP64
+direct NDFT at requested eps 1e-5; the error is printed per result.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle.kb import KBPlan
+from oracle.ndft import ndft_type1, ndft_type2, rel_l2
+from tests.gpu_util import crandn, cuda, host, rand_coords, report
+
+pytestmark = pytest.mark.gpu
+
+EPS = 1e-5
+TOL = 2e-5
+
+
+def _plans(shape, ntransf=1, tol=EPS, dtype=torch.float32, **kw):
+    from hastycucompute_amd import fft as hf
+    p1 = hf.nufft_plan.make(hf.nufft_opts(nmodes=shape, ntransf=ntransf, tol=tol, **kw),
+                            hf.nufft_type.TYPE_1, dtype)
+    p2 = hf.nufft_plan.make(hf.nufft_opts(nmodes=shape, ntransf=ntransf, tol=tol, **kw),
+                            hf.nufft_type.TYPE_2, dtype)
+    return p1, p2
+
+
+def _run(shape, coords, f, c, ntransf, **kw):
+    p1, p2 = _plans(shape, ntransf, **kw)
+    gco = [cuda(x) for x in coords]
+    p1.setpts(gco)
+    p2.setpts(gco)
+    M = len(coords[0])
+    out2 = torch.empty((ntransf, M), dtype=torch.complex64, device="cuda")
+    p2.execute(cuda(f.reshape((ntransf,) + shape)), out2)
+    out1 = torch.empty((ntransf,) + shape, dtype=torch.complex64, device="cuda")
+    p1.execute(cuda(c.reshape(ntransf, M)), out1)
+    return host(out1), host(out2)
+
+
+@pytest.mark.parametrize("shape,M", [((16, 20, 12), 3000), ((24, 24, 24), 5000), ((7, 9, 5), 1500),
+                                     ((64, 48), 4000), ((33, 20), 2500)])
+def test_type1_type2_vs_ndft(shape, M):
+    rng = np.random.default_rng(hash(shape) % 2**32)
+    coords = rand_coords(rng, len(shape), M)
+    f = crandn(rng, shape)
+    c = crandn(rng, M)
+    g1, g2 = _run(shape, coords, f, c, 1)
+    report(f"type2 {shape} M={M}", rel_l2(g2[0], ndft_type2(f, coords)), TOL)
+    report(f"type1 {shape} M={M}", rel_l2(g1[0], ndft_type1(c, coords, shape)), TOL)
+
+
+def test_batched_transforms_match_single():
+    rng = np.random.default_rng(5)
+    shape, M, T = (20, 16, 24), 4000, 3
+    coords = rand_coords(rng, 3, M)
+    f = crandn(rng, (T,) + shape)
+    c = crandn(rng, (T, M))
+    g1, g2 = _run(shape, coords, f, c, T)
+    for t in range(T):
+        report(f"batched type2 t={t}", rel_l2(g2[t], ndft_type2(f[t], coords)), TOL)
+        report(f"batched type1 t={t}", rel_l2(g1[t], ndft_type1(c[t], coords, shape)), TOL)
+
+
+def test_group_and_batch_options():
+    """Every (coil_group, max_batch, tile) option gives the same transform."""
+    rng = np.random.default_rng(6)
+    shape, M, T = (16, 16, 16), 3000, 5
+    coords = rand_coords(rng, 3, M)
+    f = crandn(rng, (T,) + shape)
+    c = crandn(rng, (T, M))
+    ref1 = ndft_type1(c, coords, shape)
+    ref2 = ndft_type2(f, coords)
+    for kw in ({"coil_group": 1}, {"coil_group": 3, "max_batch": 2}, {"tile": (8, 8, 8)},
+               {"max_subproblem": 64}):
+        g1, g2 = _run(shape, coords, f, c, T, **kw)
+        report(f"type2 opts={kw}", rel_l2(g2, ref2), TOL)
+        report(f"type1 opts={kw}", rel_l2(g1, ref1), TOL)
+
+
+def test_kooshball_vs_kb_oracle_and_ndft_subset():
+    """64^3 radial kooshball (1024 spokes x 128): full output vs the FP64 KB
+    restatement, and a point subset vs the direct NDFT."""
+    from hastycucompute_amd import synth
+    shape = (64, 64, 64)
+    coords = synth.kooshball(1024, 128)
+    M = len(coords[0])
+    f = synth.image(shape, 0)
+    rng = np.random.default_rng(9)
+    c = crandn(rng, M)
+    g1, g2 = _run(shape, coords, f, c, 1)
+    kb = KBPlan(shape, EPS).setpts(coords)
+    report("kooshball type2 vs KB oracle", rel_l2(g2[0], kb.type2(f)), 2e-6)
+    report("kooshball type1 vs KB oracle", rel_l2(g1[0], kb.type1(c)), 2e-6)
+    sel = rng.choice(M, 1500, replace=False)
+    sub = [x[sel] for x in coords]
+    report("kooshball type2 vs NDFT (subset)", rel_l2(g2[0][sel], ndft_type2(f, sub)), TOL)
+
+
+def test_special_points():
+    """k on the boundaries of [-pi, pi), at 0, and beyond (periodic)."""
+    shape = (12, 10, 8)
+    vals = np.array([-np.pi, np.pi - 1e-6, 0.0, 1e-7, -1e-7, 2.5, -3.0, 3 * np.pi / 2],
+                    np.float32)
+    rng = np.random.default_rng(3)
+    coords = [rng.permutation(vals) for _ in range(3)]
+    f = crandn(rng, shape)
+    c = crandn(rng, len(vals))
+    g1, g2 = _run(shape, coords, f, c, 1)
+    report("special points type2", rel_l2(g2[0], ndft_type2(f, coords)), TOL)
+    report("special points type1", rel_l2(g1[0], ndft_type1(c, coords, shape)), TOL)
+
+
+def test_single_point_at_origin():
+    """One point at k = 0: type 2 is sum(f), type 1 is a constant image."""
+    shape = (10, 12, 8)
+    rng = np.random.default_rng(4)
+    f = crandn(rng, shape)
+    coords = [np.zeros(1, np.float32)] * 3
+    c = np.array([2.0 - 1.0j], np.complex64)
+    g1, g2 = _run(shape, coords, f, c, 1)
+    report("origin type2 = sum f", rel_l2(g2[0], np.array([f.astype(np.complex128).sum()])), TOL)
+    report("origin type1 = const", rel_l2(g1[0], np.full(shape, c[0], np.complex128)), TOL)
+
+
+def test_empty_points():
+    shape = (8, 8, 8)
+    g1, g2 = _run(shape, [np.zeros(0, np.float32)] * 3, np.ones(shape, np.complex64),
+                  np.zeros(0, np.complex64), 1)
+    assert g2.shape == (1, 0)
+    assert np.all(g1 == 0)
+
+
+def test_adjointness():
+    rng = np.random.default_rng(11)
+    shape, M = (32, 24, 20), 20000
+    coords = rand_coords(rng, 3, M)
+    f = crandn(rng, shape)
+    c = crandn(rng, M)
+    g1, g2 = _run(shape, coords, f, c, 1)
+    lhs = np.vdot(g2[0].astype(np.complex128), c.astype(np.complex128))
+    rhs = np.vdot(f.astype(np.complex128), g1[0].astype(np.complex128))
+    err = abs(lhs - rhs) / abs(lhs)
+    report("adjointness <Af,c> = <f,A^H c>", err, 2e-5)
+
+
+def test_fp64_plan():
+    rng = np.random.default_rng(12)
+    shape, M = (12, 10, 16), 2000
+    coords = [x.astype(np.float64) for x in rand_coords(rng, 3, M)]
+    from hastycucompute_amd import fft as hf
+    p1 = hf.nufft_plan.make(hf.nufft_opts(nmodes=shape, ntransf=1, tol=1e-10), hf.nufft_type.TYPE_1,
+                            torch.float64)
+    p2 = hf.nufft_plan.make(hf.nufft_opts(nmodes=shape, ntransf=1, tol=1e-10), hf.nufft_type.TYPE_2,
+                            torch.float64)
+    gco = [cuda(x) for x in coords]
+    p1.setpts(gco)
+    p2.setpts(gco)
+    c = crandn(rng, (1, M), np.complex128)
+    f = crandn(rng, (1,) + shape, np.complex128)
+    o1 = torch.empty((1,) + shape, dtype=torch.complex128, device="cuda")
+    o2 = torch.empty((1, M), dtype=torch.complex128, device="cuda")
+    p1.execute(cuda(c), o1)
+    p2.execute(cuda(f), o2)
+    report("fp64 type1 tol 1e-10", rel_l2(host(o1)[0], ndft_type1(c[0], coords, shape)), 1e-9)
+    report("fp64 type2 tol 1e-10", rel_l2(host(o2)[0], ndft_type2(f[0], coords)), 1e-9)
+
+
+def test_plan_misuse_raises():
+    from hastycucompute_amd import fft as hf
+    p = hf.nufft_plan.make(hf.nufft_opts(nmodes=(8, 8, 8)), hf.nufft_type.TYPE_2)
+    with pytest.raises(RuntimeError):
+        p.execute(torch.zeros((1, 8, 8, 8), dtype=torch.complex64, device="cuda"),
+                  torch.zeros((1, 4), dtype=torch.complex64, device="cuda"))
+    with pytest.raises(RuntimeError):
+        p.setpts([torch.zeros(4, device="cuda"), torch.zeros(5, device="cuda"),
+                  torch.zeros(4, device="cuda")])
+    p.setpts([torch.zeros(4, device="cuda")] * 3)
+    with pytest.raises(RuntimeError):   # wrong output shape
+        p.execute(torch.zeros((1, 8, 8, 8), dtype=torch.complex64, device="cuda"),
+                  torch.zeros((1, 5), dtype=torch.complex64, device="cuda"))
+    p.free()
+    with pytest.raises(RuntimeError):
+        p.setpts([torch.zeros(4, device="cuda")] * 3)
