@@ -185,6 +185,13 @@ def main():
     if "type1" in phases:
         res["type1"] = run_phase("type1", lambda: p1.execute(y, back))
 
+    # a plain copy of known size, for scaling rocprof's FETCH_SIZE / WRITE_SIZE
+    calib = torch.empty(1 << 27, dtype=torch.complex64, device=dev)   # 1 GiB
+    calib_dst = torch.empty_like(calib)
+    calib_dst.copy_(calib)
+    torch.cuda.synchronize()
+    del calib, calib_dst
+
     # per-kernel HIP-event timing (library stream), 3 launches of each op
     _lib.profile_enable(True)
     for _ in range(3):

@@ -18,7 +18,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhasty_hip.so")
-SOURCES = ["fft.hip", "nufft.hip", "toeplitz.hip", "cg.hip", "abi.cpp"]
+SOURCES = ["fft.hip", "fft_reg_f32.hip", "fft_reg_f64.hip", "nufft.hip", "nufft_kern_f32.hip",
+           "nufft_kern_f64.hip", "toeplitz.hip", "cg.hip", "abi.cpp"]
 ARCH = os.environ.get("HASTY_OFFLOAD_ARCH", "gfx950")
 
 

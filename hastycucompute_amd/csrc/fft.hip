@@ -7,6 +7,9 @@
 #include <vector>
 
 #include "fft.hpp"
+#include "fft_reg.hpp"
+
+#include <cstdlib>
 
 namespace hasty {
 
@@ -85,8 +88,39 @@ void allow_lds(K kernel, size_t bytes) {
     });
 }
 
+// The register two-level kernels handle the common lengths; the LDS
+// Stockham kernels cover every other 2^a 3^b 5^c length.  HASTY_FFT_STOCKHAM=1
+// forces the Stockham kernels (used by the tests to cover both).
+static bool force_stockham() {
+    static const bool v = [] {
+        const char* e = std::getenv("HASTY_FFT_STOCKHAM");
+        return e && e[0] == '1';
+    }();
+    return v;
+}
+
+template <class V>
+static PassArgs<V> blank_pass() {
+    PassArgs<V> p{};
+    p.scale = 1;
+    p.ny = 1;
+    return p;
+}
+
 template <class V>
 void launch_row_load(RowLoadArgs<V> a, hipStream_t s) {
+    if (!force_stockham()) {
+        PassArgs<V> p = blank_pass<V>();
+        p.in = a.in; p.out = a.out;
+        p.in_b = a.in_b; p.in_o = 0; p.in_i = a.in_r; p.in_s = 1;
+        p.out_b = a.out_b; p.out_o = 0; p.out_i = a.out_r; p.out_s = 1;
+        p.ni = int(a.nrows); p.no = 1; p.nbatch = a.nbatch;
+        p.in_len = a.in_len; p.in_shift = a.in_shift; p.out_len = a.L; p.out_shift = 0;
+        p.lm1 = a.m1; p.lm2 = a.m2;
+        p.dz = a.dz; p.dy = a.dy; p.dx = a.dx; p.ny = a.ny; p.dz_on_load = 1;
+        p.tw = a.tw;
+        if (launch_fft2_pass<V>(p, a.L, a.sign, PM_ROW_LOAD, a.tag ? a.tag : "row_load", s)) return;
+    }
     const int64_t lines = a.nrows * a.nbatch;
     if (lines == 0) return;
     const size_t bytes = size_t(2) * a.nl * a.LP * sizeof(V);
@@ -99,6 +133,20 @@ void launch_row_load(RowLoadArgs<V> a, hipStream_t s) {
 
 template <class V>
 void launch_row_store(RowStoreArgs<V> a, hipStream_t s) {
+    if (!force_stockham()) {
+        PassArgs<V> p = blank_pass<V>();
+        p.in = a.in; p.out = a.out;
+        p.in_b = a.in_b; p.in_o = 0; p.in_i = a.in_r; p.in_s = 1;
+        p.out_b = a.out_b; p.out_o = 0; p.out_i = a.out_r; p.out_s = 1;
+        p.ni = int(a.nrows); p.no = 1; p.nbatch = a.nbatch;
+        p.in_len = a.L; p.in_shift = 0; p.out_len = a.out_len; p.out_shift = a.out_shift;
+        p.scale = a.scale;
+        p.sm1 = a.m1; p.sm2 = a.m2; p.sio = a.io;
+        p.dz = a.dz; p.dy = a.dy; p.dx = a.dx; p.ny = a.ny; p.dz_on_load = 0;
+        p.accumulate = a.accumulate; p.sum_batch = a.sum_batch;
+        p.tw = a.tw;
+        if (launch_fft2_pass<V>(p, a.L, a.sign, PM_ROW_STORE, a.tag ? a.tag : "row_store", s)) return;
+    }
     const int64_t units = a.sum_batch ? a.nrows : a.nrows * a.nbatch;
     if (units == 0) return;
     HASTY_REQUIRE(int64_t(a.nl) * a.out_len <= int64_t(kRowAccMax) * kFFTThreads,
@@ -113,6 +161,18 @@ void launch_row_store(RowStoreArgs<V> a, hipStream_t s) {
 
 template <class V>
 void launch_col_pass(ColArgs<V> a, hipStream_t s) {
+    if (!force_stockham()) {
+        PassArgs<V> p = blank_pass<V>();
+        p.in = a.in; p.out = a.out;
+        p.in_b = a.in_b; p.in_o = a.in_o; p.in_i = 1; p.in_s = a.in_s;
+        p.out_b = a.out_b; p.out_o = a.out_o; p.out_i = 1; p.out_s = a.out_s;
+        p.ni = a.ni; p.no = a.no; p.nbatch = a.nbatch;
+        p.in_len = a.in_len; p.in_shift = a.in_shift; p.out_len = a.out_len; p.out_shift = a.out_shift;
+        p.scale = a.scale;
+        p.psf = a.psf; p.psf_o = a.psf_o; p.psf_i = 1; p.psf_s = a.psf_s;
+        p.tw = a.tw;
+        if (launch_fft2_pass<V>(p, a.L, a.sign, a.psf ? PM_COL_PSF : PM_COL, a.tag ? a.tag : "col_pass", s)) return;
+    }
     const int64_t blocks = ceil_div(a.ni, a.tx) * int64_t(a.no) * a.nbatch;
     if (blocks == 0) return;
     const size_t bytes = size_t(2) * a.tx * a.LP * sizeof(V);

@@ -9,6 +9,9 @@
 
 #include "kb.hpp"
 #include "nufft.hpp"
+#include "nufft_kernels.hpp"
+
+#include <hipcub/hipcub.hpp>
 
 namespace hasty {
 
@@ -145,14 +148,6 @@ template int rows_per_block<double2>(int, int);
 // ===========================================================================
 // Point folding and bin sort
 // ===========================================================================
-struct Geo {
-    int64_t nf[3];
-    int T[3];
-    int ntile[3];
-    int wa[3];        // support width per slot (1 on a degenerate slot)
-    double half_w;
-};
-
 template <class CT>
 struct FoldArgs {
     const CT* coord[3];   // per slot; nullptr on a degenerate slot
@@ -193,14 +188,22 @@ __device__ inline int second_tile(const Geo& g, int s, int l0w) {
     return (l0w % g.T[s]) + g.wa[s] > g.T[s] ? 1 : 0;
 }
 
+// Sort key of a point: its tile, then its first support cell inside the
+// tile (so the points of a tile are grouped by cell).
+__device__ inline int64_t cell_key(const Geo& g, const int l0w[3], const int t3[3]) {
+    const int64_t cpt = int64_t(g.T[0]) * g.T[1] * g.T[2];
+    const int lz = l0w[0] - t3[0] * g.T[0], ly = l0w[1] - t3[1] * g.T[1], lx = l0w[2] - t3[2] * g.T[2];
+    return int64_t(tile_id(g, t3[0], t3[1], t3[2])) * cpt + (int64_t(lz) * g.T[1] + ly) * g.T[2] + lx;
+}
+
 template <class CT>
-__global__ void fold_count_kernel(FoldArgs<CT> a, int* pcount, int* vcount) {
+__global__ void fold_count_kernel(FoldArgs<CT> a, int* ccount, int* vcount) {
     const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (j >= a.M) return;
     int l0w[3], t3[3];
     double fr[3];
     fold_point(a, j, l0w, fr, t3);
-    atomicAdd(&pcount[tile_id(a.g, t3[0], t3[1], t3[2])], 1);
+    atomicAdd(&ccount[cell_key(a.g, l0w, t3)], 1);
     const int e0 = second_tile(a.g, 0, l0w[0]), e1 = second_tile(a.g, 1, l0w[1]),
               e2 = second_tile(a.g, 2, l0w[2]);
     for (int bz = 0; bz <= e0; ++bz)
@@ -213,19 +216,25 @@ __global__ void fold_count_kernel(FoldArgs<CT> a, int* pcount, int* vcount) {
 }
 
 template <class CT, class R>
-__global__ void fold_scatter_kernel(FoldArgs<CT> a, int* pcursor, int* vcursor, int* sidx,
-                                    int* sl0, R* sfrac, int* visits) {
+__global__ void fold_scatter_kernel(FoldArgs<CT> a, int* ccursor, int* vcursor, int* sidx,
+                                    int* sl0, R* sfrac, R* sw, int w, int ws, double beta,
+                                    int4* visits) {
     const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (j >= a.M) return;
     int l0w[3], t3[3];
     double fr[3];
     fold_point(a, j, l0w, fr, t3);
-    const int pos = atomicAdd(&pcursor[tile_id(a.g, t3[0], t3[1], t3[2])], 1);
+    const int pos = atomicAdd(&ccursor[cell_key(a.g, l0w, t3)], 1);
     sidx[pos] = int(j);
+    const double ihw = 2.0 / double(w);
     for (int s = 0; s < 3; ++s) {
         sl0[int64_t(s) * a.M + pos] = l0w[s];
         sfrac[int64_t(s) * a.M + pos] = R(fr[s]);
+        R* wp = sw + int64_t(pos) * ws + s * w;
+        for (int q = 0; q < w; ++q)
+            wp[q] = a.g.wa[s] == 1 ? R(q == 0 ? 1.0 : 0.0) : R(kb_phi<double>(fr[s] - double(q), ihw, beta));
     }
+    for (int q = 3 * w; q < ws; ++q) sw[int64_t(pos) * ws + q] = R(0);
     const int e0 = second_tile(a.g, 0, l0w[0]), e1 = second_tile(a.g, 1, l0w[1]),
               e2 = second_tile(a.g, 2, l0w[2]);
     for (int bz = 0; bz <= e0; ++bz)
@@ -234,345 +243,27 @@ __global__ void fold_scatter_kernel(FoldArgs<CT> a, int* pcursor, int* vcursor, 
                 const int t = tile_id(a.g, (t3[0] + bz) % a.g.ntile[0],
                                       (t3[1] + by) % a.g.ntile[1], (t3[2] + bx) % a.g.ntile[2]);
                 const int v = atomicAdd(&vcursor[t], 1);
-                visits[v] = pos * 8 + (bz << 2 | by << 1 | bx);
+                visits[v] = make_int4(pos * 8 + (bz << 2 | by << 1 | bx), l0w[0], l0w[1], l0w[2]);
             }
 }
 
-// ===========================================================================
-// Interpolation (type 2): one block per (tile subproblem, transform group).
-// The block stages the tile's padded footprint [t0, t0 + T + w - 1) of every
-// transform in its group into LDS, then each thread gathers w^d cells per
-// point from LDS.
-// ===========================================================================
-template <class V>
-struct GridArgs {
-    V* fine;                // [B, nf]
-    V* cdata;               // [B, M]
-    const int* sidx;
-    const int* sl0;
-    const decltype(V::x)* sfrac;
-    const int4* subs;
-    int64_t M;
-    int nbatch, group;
-    Geo g;
-    decltype(V::x) beta, inv_half_w;
-};
-
-__device__ inline void tile_origin(const Geo& g, int tile, int t0[3]) {
-    const int tx = tile % g.ntile[2];
-    const int ty = (tile / g.ntile[2]) % g.ntile[1];
-    const int tz = tile / (g.ntile[2] * g.ntile[1]);
-    t0[0] = tz * g.T[0];
-    t0[1] = ty * g.T[1];
-    t0[2] = tx * g.T[2];
+__global__ void zero_u32_kernel(unsigned* p) {
+    if (threadIdx.x == 0) p[0] = 0u;
 }
 
-template <class V, int W>
-__global__ void __launch_bounds__(kNufftThreads) interp_kernel(GridArgs<V> a) {
-    using R = decltype(V::x);
-    extern __shared__ unsigned char smem_raw[];
-    V* lds = reinterpret_cast<V*>(smem_raw);
-    const int4 sub = a.subs[blockIdx.x];
-    int t0[3];
-    tile_origin(a.g, sub.x, t0);
-    const int Pz = a.g.T[0] + a.g.wa[0] - 1;
-    const int Py = a.g.T[1] + a.g.wa[1] - 1;
-    const int Px = a.g.T[2] + a.g.wa[2] - 1;
-    const int cells = Pz * Py * Px;
-    const int g0 = blockIdx.y * a.group;
-    const int ng = min(a.group, a.nbatch - g0);
-    const int64_t G = a.g.nf[0] * a.g.nf[1] * a.g.nf[2];
-    for (int idx = threadIdx.x; idx < ng * cells; idx += blockDim.x) {
-        const int gi = idx / cells;
-        const int c = idx - gi * cells;
-        const int px = c % Px;
-        const int py = (c / Px) % Py;
-        const int pz = c / (Px * Py);
-        int gz = t0[0] + pz; gz -= gz >= a.g.nf[0] ? int(a.g.nf[0]) : 0;
-        int gy = t0[1] + py; gy -= gy >= a.g.nf[1] ? int(a.g.nf[1]) : 0;
-        int gx = t0[2] + px; gx -= gx >= a.g.nf[2] ? int(a.g.nf[2]) : 0;
-        lds[idx] = a.fine[int64_t(g0 + gi) * G + (int64_t(gz) * a.g.nf[1] + gy) * a.g.nf[2] + gx];
-    }
-    __syncthreads();
-    const int wz = a.g.wa[0];
-    for (int s = sub.y + threadIdx.x; s < sub.y + sub.z; s += blockDim.x) {
-        const int lz = a.sl0[s] - t0[0];
-        const int ly = a.sl0[a.M + s] - t0[1];
-        const int lx = a.sl0[2 * a.M + s] - t0[2];
-        const R fz = a.sfrac[s], fy = a.sfrac[a.M + s], fx = a.sfrac[2 * a.M + s];
-        R wy[W], wx[W];
-#pragma unroll
-        for (int q = 0; q < W; ++q) {
-            wy[q] = (a.g.wa[1] == 1) ? R(q == 0) : kb_phi(fy - R(q), a.inv_half_w, a.beta);
-            wx[q] = kb_phi(fx - R(q), a.inv_half_w, a.beta);
-        }
-        V acc[kMaxGroup];
-#pragma unroll
-        for (int gi = 0; gi < kMaxGroup; ++gi) acc[gi] = cmake<V>(R(0), R(0));
-        for (int qz = 0; qz < wz; ++qz) {
-            const R wzq = (wz == 1) ? R(1) : kb_phi(fz - R(qz), a.inv_half_w, a.beta);
-#pragma unroll
-            for (int gi = 0; gi < kMaxGroup; ++gi) {
-                if (gi < ng) {
-                    const V* base = lds + gi * cells + ((lz + qz) * Py + ly) * Px + lx;
-                    V accy = cmake<V>(R(0), R(0));
-#pragma unroll
-                    for (int qy = 0; qy < W; ++qy) {
-                        if (qy < a.g.wa[1]) {
-                            V accx = cmake<V>(R(0), R(0));
-#pragma unroll
-                            for (int qx = 0; qx < W; ++qx) {
-                                const V v = base[qy * Px + qx];
-                                accx.x += wx[qx] * v.x;
-                                accx.y += wx[qx] * v.y;
-                            }
-                            accy.x += wy[qy] * accx.x;
-                            accy.y += wy[qy] * accx.y;
-                        }
-                    }
-                    acc[gi].x += wzq * accy.x;
-                    acc[gi].y += wzq * accy.y;
-                }
-            }
-        }
-        const int orig = a.sidx[s];
-#pragma unroll
-        for (int gi = 0; gi < kMaxGroup; ++gi)
-            if (gi < ng) a.cdata[int64_t(g0 + gi) * a.M + orig] = acc[gi];
-    }
-}
-
-// ===========================================================================
-// Spreading (type 1): one block per (tile subproblem, transform group), the
-// block owns the tile's T_z x T_y x T_x cells.  Every point whose support
-// touches the tile (the tile's visit list) adds its clipped footprint with
-// LDS atomics; the tile is then written with plain stores, or with global
-// atomics when the tile's visits are split over several blocks (such tiles
-// are zeroed first).  No cell outside the tile is touched, so neighbouring
-// tiles never race.
-// ===========================================================================
-template <class V, int W>
-__global__ void __launch_bounds__(kNufftThreads) spread_kernel(GridArgs<V> a, const int* visits) {
-    using R = decltype(V::x);
-    extern __shared__ unsigned char smem_raw[];
-    R* lds = reinterpret_cast<R*>(smem_raw);   // [group][Tz][Ty][Tx][2]
-    const int4 sub = a.subs[blockIdx.x];
-    int t0[3];
-    tile_origin(a.g, sub.x, t0);
-    const int Tz = a.g.T[0], Ty = a.g.T[1], Tx = a.g.T[2];
-    const int cells = Tz * Ty * Tx;
-    const int g0 = blockIdx.y * a.group;
-    const int ng = min(a.group, a.nbatch - g0);
-    for (int idx = threadIdx.x; idx < 2 * ng * cells; idx += blockDim.x) lds[idx] = R(0);
-    __syncthreads();
-    for (int v = sub.y + threadIdx.x; v < sub.y + sub.z; v += blockDim.x) {
-        const int rec = visits[v];
-        const int s = rec >> 3;
-        const int bits = rec & 7;
-        // per slot: q range [qlo, qhi) and local offset (local = off + q, wrapped by T)
-        int qlo[3], qhi[3], off[3];
-        for (int d = 0; d < 3; ++d) {
-            const int l0w = a.sl0[int64_t(d) * a.M + s];
-            const int wa = a.g.wa[d];
-            if (a.g.ntile[d] == 1) {
-                qlo[d] = 0; qhi[d] = wa; off[d] = l0w;
-            } else {
-                const int r = l0w % a.g.T[d];
-                const int split = min(wa, a.g.T[d] - r);
-                const int second = (bits >> (2 - d)) & 1;
-                if (second) { qlo[d] = split; qhi[d] = wa; off[d] = -split; }
-                else { qlo[d] = 0; qhi[d] = split; off[d] = r; }
-            }
-        }
-        const R fz = a.sfrac[s], fy = a.sfrac[a.M + s], fx = a.sfrac[2 * a.M + s];
-        R wy[W], wx[W];
-#pragma unroll
-        for (int q = 0; q < W; ++q) {
-            wy[q] = (a.g.wa[1] == 1) ? R(q == 0) : kb_phi(fy - R(q), a.inv_half_w, a.beta);
-            wx[q] = kb_phi(fx - R(q), a.inv_half_w, a.beta);
-        }
-        const int orig = a.sidx[s];
-        V cv[kMaxGroup];
-#pragma unroll
-        for (int gi = 0; gi < kMaxGroup; ++gi)
-            cv[gi] = gi < ng ? a.cdata[int64_t(g0 + gi) * a.M + orig] : cmake<V>(R(0), R(0));
-        for (int qz = qlo[0]; qz < qhi[0]; ++qz) {
-            const R wzq = (a.g.wa[0] == 1) ? R(1) : kb_phi(fz - R(qz), a.inv_half_w, a.beta);
-            int lz = off[0] + qz; lz -= lz >= Tz ? Tz : 0;
-#pragma unroll
-            for (int qy = 0; qy < W; ++qy) {
-                if (qy >= qlo[1] && qy < qhi[1]) {
-                    int ly = off[1] + qy; ly -= ly >= Ty ? Ty : 0;
-                    const R wzy = wzq * wy[qy];
-#pragma unroll
-                    for (int qx = 0; qx < W; ++qx) {
-                        if (qx >= qlo[2] && qx < qhi[2]) {
-                            int lx = off[2] + qx; lx -= lx >= Tx ? Tx : 0;
-                            const R ww = wzy * wx[qx];
-                            const int cell = (lz * Ty + ly) * Tx + lx;
-#pragma unroll
-                            for (int gi = 0; gi < kMaxGroup; ++gi) {
-                                if (gi < ng) {
-                                    atomicAdd(&lds[2 * (gi * cells + cell)], ww * cv[gi].x);
-                                    atomicAdd(&lds[2 * (gi * cells + cell) + 1], ww * cv[gi].y);
-                                }
-                            }
-                        }
-                    }
-                }
-            }
-        }
-    }
-    __syncthreads();
-    const int64_t G = a.g.nf[0] * a.g.nf[1] * a.g.nf[2];
-    const bool split = sub.w != 0;
-    for (int idx = threadIdx.x; idx < ng * cells; idx += blockDim.x) {
-        const int gi = idx / cells;
-        const int c = idx - gi * cells;
-        const int lx = c % Tx;
-        const int ly = (c / Tx) % Ty;
-        const int lz = c / (Tx * Ty);
-        V* dst = a.fine + int64_t(g0 + gi) * G +
-                 (int64_t(t0[0] + lz) * a.g.nf[1] + (t0[1] + ly)) * a.g.nf[2] + (t0[2] + lx);
-        const R re = lds[2 * idx], im = lds[2 * idx + 1];
-        if (split) {
-            atomicAdd(&dst->x, re);
-            atomicAdd(&dst->y, im);
-        } else {
-            *dst = cmake<V>(re, im);
-        }
-    }
-}
-
-// ===========================================================================
-// Global-memory variants (method GM, or when a tile does not fit in LDS):
-// one thread per bin-sorted point, reading / atomically adding the fine grid
-// directly with wrapped indices.
-// ===========================================================================
-template <class V, int W>
-__global__ void __launch_bounds__(kNufftThreads) interp_gm_kernel(GridArgs<V> a) {
-    using R = decltype(V::x);
-    const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (s >= a.M) return;
-    int iz[W], iy[W], ix[W];
-    R wz[W], wy[W], wx[W];
-    const R* fr[3] = {a.sfrac, a.sfrac + a.M, a.sfrac + 2 * a.M};
-    int* idx[3] = {iz, iy, ix};
-    R* wt[3] = {wz, wy, wx};
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const int l0 = a.sl0[int64_t(d) * a.M + s];
-        const R f = fr[d][s];
-#pragma unroll
-        for (int q = 0; q < W; ++q) {
-            int l = l0 + q;
-            l -= l >= int(a.g.nf[d]) ? int(a.g.nf[d]) : 0;
-            idx[d][q] = l;
-            wt[d][q] = a.g.wa[d] == 1 ? R(q == 0) : kb_phi(f - R(q), a.inv_half_w, a.beta);
-        }
-    }
-    const int orig = a.sidx[s];
-    const int64_t G = a.g.nf[0] * a.g.nf[1] * a.g.nf[2];
-    for (int t = 0; t < a.nbatch; ++t) {
-        const V* g = a.fine + int64_t(t) * G;
-        V acc = cmake<V>(R(0), R(0));
-        for (int qz = 0; qz < a.g.wa[0]; ++qz) {
-            V accy = cmake<V>(R(0), R(0));
-            for (int qy = 0; qy < a.g.wa[1]; ++qy) {
-                const V* row = g + (int64_t(iz[qz]) * a.g.nf[1] + iy[qy]) * a.g.nf[2];
-                V accx = cmake<V>(R(0), R(0));
-#pragma unroll
-                for (int qx = 0; qx < W; ++qx) {
-                    const V v = row[ix[qx]];
-                    accx.x += wx[qx] * v.x;
-                    accx.y += wx[qx] * v.y;
-                }
-                accy.x += wy[qy] * accx.x;
-                accy.y += wy[qy] * accx.y;
-            }
-            acc.x += wz[qz] * accy.x;
-            acc.y += wz[qz] * accy.y;
-        }
-        a.cdata[int64_t(t) * a.M + orig] = acc;
-    }
-}
-
-template <class V, int W>
-__global__ void __launch_bounds__(kNufftThreads) spread_gm_kernel(GridArgs<V> a) {
-    using R = decltype(V::x);
-    const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (s >= a.M) return;
-    int iz[W], iy[W], ix[W];
-    R wz[W], wy[W], wx[W];
-    const R* fr[3] = {a.sfrac, a.sfrac + a.M, a.sfrac + 2 * a.M};
-    int* idx[3] = {iz, iy, ix};
-    R* wt[3] = {wz, wy, wx};
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const int l0 = a.sl0[int64_t(d) * a.M + s];
-        const R f = fr[d][s];
-#pragma unroll
-        for (int q = 0; q < W; ++q) {
-            int l = l0 + q;
-            l -= l >= int(a.g.nf[d]) ? int(a.g.nf[d]) : 0;
-            idx[d][q] = l;
-            wt[d][q] = a.g.wa[d] == 1 ? R(q == 0) : kb_phi(f - R(q), a.inv_half_w, a.beta);
-        }
-    }
-    const int orig = a.sidx[s];
-    const int64_t G = a.g.nf[0] * a.g.nf[1] * a.g.nf[2];
-    for (int t = 0; t < a.nbatch; ++t) {
-        V* g = a.fine + int64_t(t) * G;
-        const V c = a.cdata[int64_t(t) * a.M + orig];
-        for (int qz = 0; qz < a.g.wa[0]; ++qz) {
-            for (int qy = 0; qy < a.g.wa[1]; ++qy) {
-                V* row = g + (int64_t(iz[qz]) * a.g.nf[1] + iy[qy]) * a.g.nf[2];
-                const R wzy = wz[qz] * wy[qy];
-#pragma unroll
-                for (int qx = 0; qx < W; ++qx) {
-                    const R ww = wzy * wx[qx];
-                    atomicAdd(&row[ix[qx]].x, ww * c.x);
-                    atomicAdd(&row[ix[qx]].y, ww * c.y);
-                }
-            }
-        }
-    }
-}
-
-template <class V>
-__global__ void fill_zero_kernel(V* p, int64_t n) {
-    using R = decltype(V::x);
-    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-         i += int64_t(gridDim.x) * blockDim.x)
-        p[i] = cmake<V>(R(0), R(0));
-}
-
-// Zero the cells of split tiles before their blocks accumulate atomically.
-template <class V>
-__global__ void zero_tiles_kernel(V* fine, const int* tiles, int ntiles_split, int nbatch, Geo g) {
-    using R = decltype(V::x);
-    const int cells = g.T[0] * g.T[1] * g.T[2];
-    const int64_t G = g.nf[0] * g.nf[1] * g.nf[2];
-    const int tile = tiles[blockIdx.x];
-    int t0[3];
-    tile_origin(g, tile, t0);
-    for (int idx = threadIdx.x; idx < nbatch * cells; idx += blockDim.x) {
-        const int b = idx / cells;
-        const int c = idx - b * cells;
-        const int lx = c % g.T[2];
-        const int ly = (c / g.T[2]) % g.T[1];
-        const int lz = c / (g.T[2] * g.T[1]);
-        fine[int64_t(b) * G + (int64_t(t0[0] + lz) * g.nf[1] + (t0[1] + ly)) * g.nf[2] + (t0[2] + lx)] =
-            cmake<V>(R(0), R(0));
-    }
+// toff[t] = first sorted point of tile t (t <= ntiles) from the cell offsets.
+__global__ void tile_offsets_kernel(const int* coff, int64_t cpt, int64_t ntiles, int M, int* toff) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t < ntiles) toff[t] = coff[t * cpt];
+    if (t == ntiles) toff[t] = M;
 }
 
 // ===========================================================================
 // Plan
 // ===========================================================================
 NufftPlan::~NufftPlan() {
-    void* ptrs[] = {deapod[0], deapod[1], deapod[2], fine, bufA, bufB, sidx, sl0, sfrac,
-                    visits, sub_interp, sub_spread, split_tiles};
+    void* ptrs[] = {deapod[0], deapod[1], deapod[2], fine, bufA, bufB, sidx, sl0, sfrac, sw, csort,
+                    visits, sub_interp, sub_spread, split_info, partial, cmax};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -600,12 +291,27 @@ template <class T> void dev_free(T*& p, int64_t bytes, int64_t& tally) {
     }
 }
 
+void dev_free_void(void*& p, int64_t bytes, int64_t& tally) {
+    if (p) {
+        HASTY_CHECK_HIP(hipFree(p));
+        tally -= bytes;
+        p = nullptr;
+    }
+}
+
+// floats per point in the weight table: 3 slots x w, padded to 16 bytes
+int weight_stride(int w) { return (3 * w + 3) / 4 * 4; }
+
+// Tile edge along one axis: the largest divisor of nf that is <= target and
+// >= w (a support then touches at most two tiles), else the smallest divisor
+// >= w.
 int pick_tile(int64_t nf, int target, int w) {
     if (nf <= 1) return 1;
-    int best = 0;
-    for (int t = std::min<int64_t>(target, nf); t >= 1; --t)
-        if (nf % t == 0 && t >= w) { best = t; break; }
-    return best ? best : int(nf);
+    for (int t = int(std::min<int64_t>(target, nf)); t >= w; --t)
+        if (nf % t == 0) return t;
+    for (int64_t t = std::max(w, 1); t < nf; ++t)
+        if (nf % t == 0) return int(t);
+    return int(nf);
 }
 
 Geo geo_of(const NufftPlan* p) {
@@ -620,52 +326,52 @@ Geo geo_of(const NufftPlan* p) {
     return g;
 }
 
-template <class V> using InterpFn = void (*)(GridArgs<V>);
-template <class V> using SpreadFn = void (*)(GridArgs<V>, const int*);
-
-#define HASTY_WIDTH_CASES(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14)
-
-template <class V, int W> struct KernelTable {
-    static void* interp() { return reinterpret_cast<void*>(&interp_kernel<V, W>); }
-    static void* spread() { return reinterpret_cast<void*>(&spread_kernel<V, W>); }
-    static void* interp_gm() { return reinterpret_cast<void*>(&interp_gm_kernel<V, W>); }
-    static void* spread_gm() { return reinterpret_cast<void*>(&spread_gm_kernel<V, W>); }
-};
-
-template <class V> void* kernel_for_width(int w, int which) {
-    switch (w) {
-#define C(Wv)                                                        \
-    case Wv:                                                         \
-        return which == 0 ? KernelTable<V, Wv>::interp()             \
-             : which == 1 ? KernelTable<V, Wv>::spread()             \
-             : which == 2 ? KernelTable<V, Wv>::interp_gm()          \
-                          : KernelTable<V, Wv>::spread_gm();
-        HASTY_WIDTH_CASES(C)
-#undef C
-    }
-    throw Error(HASTY_ERR_INVALID, "kernel width " + std::to_string(w) + " unsupported");
-}
-template <class V> void* interp_for_width(int w) { return kernel_for_width<V>(w, 0); }
-template <class V> void* spread_for_width(int w) { return kernel_for_width<V>(w, 1); }
-template <class V> void* interp_gm_for_width(int w) { return kernel_for_width<V>(w, 2); }
-template <class V> void* spread_gm_for_width(int w) { return kernel_for_width<V>(w, 3); }
-
-constexpr int kMaxWidth = 14;
-
 size_t interp_lds_bytes(const NufftPlan* p, int group) {
     size_t cells = 1;
     for (int s = 0; s < 3; ++s) cells *= size_t(p->T[s] + (p->nf[s] == 1 ? 1 : p->w) - 1);
     return cells * size_t(group) * (p->f64 ? 16 : 8);
 }
+
+// 64-bit accumulators per transform of a spreading tile: 2 per cell, padded
+// so that the stride is 2 mod 32 (lane l = 2 transform + component lands on
+// accumulator l mod 32: consecutive lanes, consecutive 8-byte slots).
+int spread_cstride(const NufftPlan* p) {
+    const int cells = p->T[0] * p->T[1] * p->T[2];
+    int cs = 2 * cells;
+    cs += ((2 - cs) % 32 + 32) % 32;
+    return cs;
+}
 size_t spread_lds_bytes(const NufftPlan* p, int group) {
-    return size_t(p->T[0]) * p->T[1] * p->T[2] * size_t(group) * (p->f64 ? 16 : 8);
+    return size_t(spread_cstride(p)) * size_t(group) * 8;
+}
+
+// peak of the window (t = 0), raised to the number of non-degenerate slots
+double window_peak_pow(const NufftPlan* p) {
+    double peak = 0.0;
+    // phi(0) = I0(beta): power series
+    double q = 0.25 * p->beta * p->beta, term = 1.0, sum = 1.0;
+    for (int k = 1; k < 200; ++k) {
+        term *= q / (double(k) * double(k));
+        sum += term;
+        if (term < 1e-17 * sum) break;
+    }
+    peak = sum;
+    double r = 1.0;
+    for (int s = 0; s < 3; ++s)
+        if (p->nf[s] > 1) r *= peak;
+    return r * 1.0000001;
 }
 
 void set_lds_limit(void* fn, size_t bytes) {
-    HASTY_CHECK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)));
+    if (bytes > 64 * 1024)
+        HASTY_CHECK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)));
 }
 
 constexpr size_t kLdsMax = 160 * 1024;
+
+void* kernel_ptr(const NufftPlan* p, int which) {
+    return p->f64 ? nufft_kernel_ptr<double2>(which, p->w) : nufft_kernel_ptr<float2>(which, p->w);
+}
 
 }  // namespace
 
@@ -683,7 +389,9 @@ NufftPlan* nufft_makeplan(int type, int dim, const int64_t* nmodes, int sign, in
     p->sigma = (opts && opts->upsamp > 0) ? opts->upsamp : 2.0;
     HASTY_REQUIRE(p->sigma >= 1.25, "upsampling factor must be >= 1.25");
     p->w = kb_width_for_tol(tol);
-    HASTY_REQUIRE(p->w <= kMaxWidth, "requested tolerance needs a kernel wider than 14");
+    HASTY_REQUIRE(p->w <= (f64 ? kMaxWidthF64 : kMaxWidthF32),
+                  f64 ? "tolerance below 1e-13 is not supported"
+                      : "complex64 plans support tolerances down to 1e-8 (kernel width 9)");
     p->beta = kb_beta(p->w, p->sigma);
     p->stream = stream;
     p->device = current_device();
@@ -695,32 +403,33 @@ NufftPlan* nufft_makeplan(int type, int dim, const int64_t* nmodes, int sign, in
         p->nf[s] = next_smooth(std::max<int64_t>(int64_t(std::ceil(p->sigma * p->n[s])), 2 * p->w));
         HASTY_REQUIRE(p->nf[s] <= (1 << 20), "fine grid too large");
     }
-    // tiles: target 16 cells per axis in 3-D, 32 in 2-D (divisors of nf, >= w)
-    const int target = dim == 3 ? 16 : 32;
+    // tiles: interpolation stages a padded 16^3 / 32^2 tile per transform;
+    // spreading owns an unpadded 8^3 / 16^2 tile of 64-bit accumulators for a
+    // group of transforms (16 / 32 of them fit in 128 KiB of LDS)
+    const int target3[3] = {type == 2 ? 16 : 8, type == 2 ? 16 : 8, type == 2 ? 16 : 8};
+    const int target2 = type == 2 ? 32 : 16;
     for (int s = 0; s < 3; ++s) {
+        const int target = dim == 3 ? target3[s] : target2;
         int want = (opts && opts->tile[s] > 0) ? opts->tile[s] : target;
         p->T[s] = p->nf[s] == 1 ? 1 : pick_tile(p->nf[s], want, p->w);
         p->ntile[s] = int(p->nf[s] / p->T[s]);
     }
     p->ntiles = int64_t(p->ntile[0]) * p->ntile[1] * p->ntile[2];
-    int group = (opts && opts->coil_group > 0) ? opts->coil_group : (f64 ? 1 : 2);
-    group = std::min(std::min(group, kMaxGroup), ntransf);
-    // LDS-tiled kernels when the tile fits (method SM / DEFAULT); the
-    // global-memory kernels for method GM_* or when it does not fit.
+    const int maxg = type == 2 ? kMaxGroupInterp : kMaxGroupSpread;
+    int group = (opts && opts->coil_group > 0) ? opts->coil_group : (type == 2 ? 1 : maxg);
+    group = std::min(std::min(group, maxg), ntransf);
     const int method = opts ? opts->method : 3;
     auto lds_of = [&](int g) {
         return type == 2 ? interp_lds_bytes(p.get(), g) : spread_lds_bytes(p.get(), g);
     };
-    while (group > 1 && lds_of(group) > kLdsMax) --group;
+    // largest power of two (or ntransf) whose tile fits
+    while (group > 1 && lds_of(group) > kLdsMax) group = (group & (group - 1)) ? (group & (group - 1)) : group / 2;
     p->gm = (method == 0 || method == 1) || lds_of(1) > kLdsMax;
     p->group = group;
+    p->cstride = spread_cstride(p.get());
     p->max_batch = (opts && opts->max_batch > 0) ? std::min(opts->max_batch, ntransf) : ntransf;
-    p->max_sub = (opts && opts->max_subproblem > 0) ? opts->max_subproblem : 8192;
-    if (!p->gm) {
-        void* fn = type == 2 ? (f64 ? interp_for_width<double2>(p->w) : interp_for_width<float2>(p->w))
-                             : (f64 ? spread_for_width<double2>(p->w) : spread_for_width<float2>(p->w));
-        set_lds_limit(fn, lds_of(group));
-    }
+    p->max_sub = (opts && opts->max_subproblem > 0) ? opts->max_subproblem : (type == 2 ? 8192 : 2048);
+    if (!p->gm) set_lds_limit(kernel_ptr(p.get(), type == 2 ? 0 : 1), lds_of(group));
     // deapodisation factors 1/Phi(2 pi m / nf) per slot, centred modes
     for (int s = 0; s < 3; ++s) {
         const int64_t n = p->n[s];
@@ -763,23 +472,31 @@ void nufft_setpts_typed(NufftPlan* p, int64_t M, const void* c0, const void* c1,
     p->have_pts = false;
     const hipStream_t s = p->stream;
     const Geo g = geo_of(p);
+    const int64_t rsz = p->f64 ? 8 : 4;
     p->M = M;
-    // grow point storage
     if (M > p->pts_cap) {
         dev_free(p->sidx, p->pts_cap * 4, p->device_bytes);
         dev_free(p->sl0, p->pts_cap * 12, p->device_bytes);
-        unsigned char* sf = static_cast<unsigned char*>(p->sfrac);
-        dev_free(sf, p->pts_cap * 3 * (p->f64 ? 8 : 4), p->device_bytes);
-        p->sfrac = nullptr;
+        dev_free_void(p->sfrac, p->pts_cap * 3 * rsz, p->device_bytes);
+        dev_free_void(p->sw, p->pts_cap * weight_stride(p->w) * rsz, p->device_bytes);
         p->sidx = dev_alloc<int>(M, p->device_bytes);
         p->sl0 = dev_alloc<int>(3 * M, p->device_bytes);
-        p->sfrac = dev_alloc<unsigned char>(3 * M * (p->f64 ? 8 : 4), p->device_bytes);
+        p->sfrac = dev_alloc<unsigned char>(3 * M * rsz, p->device_bytes);
+        p->sw = dev_alloc<unsigned char>(M * weight_stride(p->w) * rsz, p->device_bytes);
         p->pts_cap = M;
     }
     const int64_t nt = p->ntiles;
+    const int64_t cpt = int64_t(p->T[0]) * p->T[1] * p->T[2];
+    const int64_t nkeys = nt * cpt;                      // = fine-grid cells
     int64_t tally_tmp = 0;
-    int* counts = dev_alloc<int>(4 * nt, tally_tmp);   // pcount, vcount, pcursor, vcursor
-    HASTY_CHECK_HIP(hipMemsetAsync(counts, 0, size_t(4 * nt) * sizeof(int), s));
+    // ccount[nkeys] -> (scan) coff[nkeys] used as the scatter cursor;
+    // vcount / vcursor [nt]; toff [nt + 1]
+    int* ccount = dev_alloc<int>(nkeys, tally_tmp);
+    int* coff = dev_alloc<int>(nkeys, tally_tmp);
+    int* vbuf = dev_alloc<int>(2 * nt, tally_tmp);
+    int* toff = dev_alloc<int>(nt + 1, tally_tmp);
+    HASTY_CHECK_HIP(hipMemsetAsync(ccount, 0, size_t(nkeys) * sizeof(int), s));
+    HASTY_CHECK_HIP(hipMemsetAsync(vbuf, 0, size_t(2 * nt) * sizeof(int), s));
     auto run_fold = [&](auto* tag) {
         using CT = std::remove_pointer_t<decltype(tag)>;
         FoldArgs<CT> fa{};
@@ -793,43 +510,54 @@ void nufft_setpts_typed(NufftPlan* p, int64_t M, const void* c0, const void* c1,
     if (M > 0) {
         if (coords_f64) {
             auto fa = run_fold((double*)nullptr);
-            hipLaunchKernelGGL(fold_count_kernel<double>, dim3(blocks), dim3(256), 0, s, fa, counts, counts + nt);
+            hipLaunchKernelGGL(fold_count_kernel<double>, dim3(blocks), dim3(256), 0, s, fa, ccount, vbuf);
         } else {
             auto fa = run_fold((float*)nullptr);
-            hipLaunchKernelGGL(fold_count_kernel<float>, dim3(blocks), dim3(256), 0, s, fa, counts, counts + nt);
+            hipLaunchKernelGGL(fold_count_kernel<float>, dim3(blocks), dim3(256), 0, s, fa, ccount, vbuf);
         }
         HASTY_CHECK_HIP(hipGetLastError());
     }
-    std::vector<int> hcount(size_t(2 * nt));
-    HASTY_CHECK_HIP(hipMemcpyAsync(hcount.data(), counts, size_t(2 * nt) * sizeof(int),
-                                   hipMemcpyDeviceToHost, s));
+    {
+        size_t tmp_bytes = 0;
+        HASTY_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, ccount, coff, nkeys, s));
+        void* tmp = dev_alloc<unsigned char>(int64_t(std::max<size_t>(tmp_bytes, 1)), tally_tmp);
+        HASTY_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, ccount, coff, nkeys, s));
+        hipLaunchKernelGGL(tile_offsets_kernel, dim3(unsigned(ceil_div(nt + 1, 256))), dim3(256), 0, s,
+                           static_cast<const int*>(coff), cpt, nt, int(M), toff);
+        HASTY_CHECK_HIP(hipGetLastError());
+        HASTY_CHECK_HIP(hipStreamSynchronize(s));
+        HASTY_CHECK_HIP(hipFree(tmp));
+    }
+    std::vector<int> htoff(static_cast<size_t>(nt + 1)), hvcount(static_cast<size_t>(nt));
+    HASTY_CHECK_HIP(hipMemcpyAsync(htoff.data(), toff, size_t(nt + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
+    HASTY_CHECK_HIP(hipMemcpyAsync(hvcount.data(), vbuf, size_t(nt) * sizeof(int), hipMemcpyDeviceToHost, s));
     HASTY_CHECK_HIP(hipStreamSynchronize(s));
-    // offsets and subproblems
-    std::vector<int> cursors(size_t(2 * nt));
-    std::vector<int4> isubs, ssubs;
-    std::vector<int> splits;
-    int64_t poff = 0, voff = 0;
+    // subproblems; split tiles get one partial slot per subproblem
+    std::vector<int> vcursors(static_cast<size_t>(nt));
+    std::vector<int4> isubs, ssubs, splits;
+    int64_t voff = 0;
+    int nslots = 0;
     for (int64_t t = 0; t < nt; ++t) {
-        const int pc = hcount[t], vc = hcount[nt + t];
-        cursors[t] = int(poff);
-        cursors[nt + t] = int(voff);
+        const int poff = htoff[t], pc = htoff[t + 1] - htoff[t], vc = hvcount[t];
+        vcursors[t] = int(voff);
         for (int st = 0; st < pc; st += p->max_sub)
-            isubs.push_back(make_int4(int(t), int(poff + st), std::min(p->max_sub, pc - st), 0));
+            isubs.push_back(make_int4(int(t), poff + st, std::min(p->max_sub, pc - st), 0));
         if (vc <= p->max_sub) {
             ssubs.push_back(make_int4(int(t), int(voff), vc, 0));
         } else {
-            splits.push_back(int(t));
+            const int first = nslots;
             for (int st = 0; st < vc; st += p->max_sub)
-                ssubs.push_back(make_int4(int(t), int(voff + st), std::min(p->max_sub, vc - st), 1));
+                ssubs.push_back(make_int4(int(t), int(voff + st), std::min(p->max_sub, vc - st), 1 + nslots++));
+            splits.push_back(make_int4(int(t), first, nslots - first, 0));
         }
-        poff += pc;
         voff += vc;
     }
-    HASTY_REQUIRE(poff == M, "internal: bin counts do not sum to M");
+    HASTY_REQUIRE(htoff[nt] == M, "internal: bin counts do not sum to M");
     p->nvisits = voff;
+    p->nslots = nslots;
     if (voff > p->visits_cap) {
-        dev_free(p->visits, p->visits_cap * 4, p->device_bytes);
-        p->visits = dev_alloc<int>(voff, p->device_bytes);
+        dev_free(p->visits, p->visits_cap * 16, p->device_bytes);
+        p->visits = dev_alloc<int4>(voff, p->device_bytes);
         p->visits_cap = voff;
     }
     auto upload_int4 = [&](std::vector<int4>& v, int4*& ptr, int64_t& cap, int& n) {
@@ -844,16 +572,8 @@ void nufft_setpts_typed(NufftPlan* p, int64_t M, const void* c0, const void* c1,
     };
     upload_int4(isubs, p->sub_interp, p->sub_interp_cap, p->nsub_interp);
     upload_int4(ssubs, p->sub_spread, p->sub_spread_cap, p->nsub_spread);
-    p->nsplit = int(splits.size());
-    if (int64_t(splits.size()) > p->split_cap) {
-        dev_free(p->split_tiles, p->split_cap * 4, p->device_bytes);
-        p->split_tiles = dev_alloc<int>(int64_t(splits.size()), p->device_bytes);
-        p->split_cap = int64_t(splits.size());
-    }
-    if (!splits.empty())
-        HASTY_CHECK_HIP(hipMemcpyAsync(p->split_tiles, splits.data(), splits.size() * sizeof(int),
-                                       hipMemcpyHostToDevice, s));
-    HASTY_CHECK_HIP(hipMemcpyAsync(counts + 2 * nt, cursors.data(), size_t(2 * nt) * sizeof(int),
+    upload_int4(splits, p->split_info, p->split_cap, p->nsplit);
+    HASTY_CHECK_HIP(hipMemcpyAsync(vbuf + nt, vcursors.data(), size_t(nt) * sizeof(int),
                                    hipMemcpyHostToDevice, s));
     if (M > 0) {
         auto scatter = [&](auto* ctag, auto* rtag) {
@@ -861,8 +581,9 @@ void nufft_setpts_typed(NufftPlan* p, int64_t M, const void* c0, const void* c1,
             using RT = std::remove_pointer_t<decltype(rtag)>;
             auto fa = run_fold((CT*)nullptr);
             hipLaunchKernelGGL((fold_scatter_kernel<CT, RT>), dim3(blocks), dim3(256), 0, s, fa,
-                               counts + 2 * nt, counts + 3 * nt, p->sidx, p->sl0,
-                               static_cast<RT*>(p->sfrac), p->visits);
+                               coff, vbuf + nt, p->sidx, p->sl0,
+                               static_cast<RT*>(p->sfrac), static_cast<RT*>(p->sw), p->w,
+                               weight_stride(p->w), p->beta, p->visits);
         };
         if (coords_f64 && p->f64) scatter((double*)nullptr, (double*)nullptr);
         else if (coords_f64) scatter((double*)nullptr, (float*)nullptr);
@@ -872,7 +593,10 @@ void nufft_setpts_typed(NufftPlan* p, int64_t M, const void* c0, const void* c1,
     }
     // the host vectors above are copied asynchronously: wait before they go
     HASTY_CHECK_HIP(hipStreamSynchronize(s));
-    HASTY_CHECK_HIP(hipFree(counts));
+    HASTY_CHECK_HIP(hipFree(ccount));
+    HASTY_CHECK_HIP(hipFree(coff));
+    HASTY_CHECK_HIP(hipFree(vbuf));
+    HASTY_CHECK_HIP(hipFree(toff));
     p->have_pts = true;
 }
 
@@ -881,6 +605,8 @@ void nufft_setpts(NufftPlan* p, int64_t M, const void* c0, const void* c1, const
 }
 
 namespace {
+
+template <class V> using GridFn = void (*)(GridArgs<V>);
 
 template <class V>
 void execute_impl(NufftPlan* p, V* c, V* f) {
@@ -901,59 +627,95 @@ void execute_impl(NufftPlan* p, V* c, V* f) {
     V* fine = static_cast<V*>(p->fine);
     V* bufA = static_cast<V*>(p->bufA);
     V* bufB = static_cast<V*>(p->bufB);
+    if (p->type == 1 && !p->gm) {
+        // the batch's max |component| (fixed-point scale of the spreading);
+        // allocated even for M == 0 since every tile block reads it
+        if (!p->cmax) p->cmax = dev_alloc<unsigned>(1, p->device_bytes);
+    }
+    if (p->type == 1 && !p->gm && p->M > 0) {
+        const int64_t need = p->M * int64_t(p->max_batch);
+        if (need > p->csort_cap) {
+            dev_free_void(p->csort, p->csort_cap * int64_t(sizeof(V)), p->device_bytes);
+            p->csort = dev_alloc<V>(need, p->device_bytes);
+            p->csort_cap = need;
+        }
+        const int64_t pneed = int64_t(p->nslots) * p->max_batch * p->T[0] * p->T[1] * p->T[2];
+        if (pneed > p->partial_cap) {
+            dev_free_void(p->partial, p->partial_cap * int64_t(sizeof(V)), p->device_bytes);
+            p->partial = dev_alloc<V>(pneed, p->device_bytes);
+            p->partial_cap = pneed;
+        }
+    }
     for (int b0 = 0; b0 < p->ntransf; b0 += p->max_batch) {
         const int nb = std::min(p->max_batch, p->ntransf - b0);
         V* cb = c + int64_t(b0) * p->M;
         V* fb = f + int64_t(b0) * nimg;
         GridArgs<V> ga{};
-        ga.fine = fine; ga.cdata = cb; ga.sidx = p->sidx; ga.sl0 = p->sl0;
-        ga.sfrac = static_cast<const R*>(p->sfrac);
-        ga.M = p->M; ga.nbatch = nb; ga.group = p->group; ga.g = g;
-        ga.beta = R(p->beta); ga.inv_half_w = R(2.0 / p->w);
-        const unsigned ngroups = unsigned(ceil_div(nb, p->group));
+        ga.fine = fine; ga.cdata = cb; ga.csort = static_cast<const V*>(p->csort);
+        ga.sidx = p->sidx; ga.sl0 = p->sl0; ga.sw = static_cast<const R*>(p->sw);
+        ga.visits = p->visits;
+        ga.M = p->M; ga.nbatch = nb; ga.group = std::min(p->group, nb); ga.w = p->w;
+        ga.ws = weight_stride(p->w);
+        ga.cstride = p->cstride; ga.g = g;
+        ga.partial = static_cast<V*>(p->partial);
+        ga.cmax = p->cmax;
+        ga.wmax3 = window_peak_pow(p);
+        const unsigned ngroups = unsigned(ceil_div(nb, ga.group));
         if (p->type == 2) {
             DiagOp<V> none{};
             pruned_forward<V>(pg, nb, fb, fine, bufA, bufB, p->sign, none, none, deapod, s);
-            if (p->gm && p->M > 0) {
-                void* fn = interp_gm_for_width<V>(p->w);
+            if (p->M == 0) continue;
+            if (p->gm) {
                 HASTY_PROF("nufft_interp_gm", s);
-                hipLaunchKernelGGL(reinterpret_cast<InterpFn<V>>(fn), dim3(unsigned(ceil_div(p->M, kNufftThreads))),
-                                   dim3(kNufftThreads), 0, s, ga);
-                HASTY_CHECK_HIP(hipGetLastError());
-            } else if (p->nsub_interp > 0) {
+                hipLaunchKernelGGL(reinterpret_cast<GridFn<V>>(kernel_ptr(p, 2)),
+                                   dim3(unsigned(ceil_div(p->M, 256))), dim3(256), 0, s, ga);
+            } else {
                 ga.subs = p->sub_interp;
-                void* fn = interp_for_width<V>(p->w);
                 HASTY_PROF("nufft_interp", s);
-                hipLaunchKernelGGL(reinterpret_cast<InterpFn<V>>(fn), dim3(unsigned(p->nsub_interp), ngroups),
-                                   dim3(kNufftThreads), interp_lds_bytes(p, p->group), s, ga);
-                HASTY_CHECK_HIP(hipGetLastError());
+                hipLaunchKernelGGL(reinterpret_cast<GridFn<V>>(kernel_ptr(p, 0)),
+                                   dim3(unsigned(p->nsub_interp), ngroups), dim3(kNufftThreads),
+                                   interp_lds_bytes(p, ga.group), s, ga);
             }
+            HASTY_CHECK_HIP(hipGetLastError());
         } else {
             if (p->gm) {
                 HASTY_PROF("nufft_spread_gm", s);
-                hipLaunchKernelGGL(fill_zero_kernel<V>, dim3(unsigned(std::min<int64_t>(8192, ceil_div(nb * nfine, 256)))),
+                hipLaunchKernelGGL(fill_zero_kernel<V>,
+                                   dim3(unsigned(std::min<int64_t>(8192, ceil_div(nb * nfine, 256)))),
                                    dim3(256), 0, s, fine, int64_t(nb) * nfine);
-                if (p->M > 0) {
-                    void* fn = spread_gm_for_width<V>(p->w);
-                    hipLaunchKernelGGL(reinterpret_cast<InterpFn<V>>(fn), dim3(unsigned(ceil_div(p->M, kNufftThreads))),
-                                       dim3(kNufftThreads), 0, s, ga);
-                }
+                if (p->M > 0)
+                    hipLaunchKernelGGL(reinterpret_cast<GridFn<V>>(kernel_ptr(p, 3)),
+                                       dim3(unsigned(ceil_div(p->M, 256))), dim3(256), 0, s, ga);
                 HASTY_CHECK_HIP(hipGetLastError());
             } else {
-            if (p->nsplit > 0) {
-                hipLaunchKernelGGL(zero_tiles_kernel<V>, dim3(unsigned(p->nsplit)), dim3(256), 0, s, fine,
-                                   p->split_tiles, p->nsplit, nb, g);
-                HASTY_CHECK_HIP(hipGetLastError());
-            }
-            ga.subs = p->sub_spread;
-            void* fn = spread_for_width<V>(p->w);
-            {
-            HASTY_PROF("nufft_spread", s);
-            hipLaunchKernelGGL(reinterpret_cast<SpreadFn<V>>(fn), dim3(unsigned(p->nsub_spread), ngroups),
-                               dim3(kNufftThreads), spread_lds_bytes(p, p->group), s, ga,
-                               static_cast<const int*>(p->visits));
-            HASTY_CHECK_HIP(hipGetLastError());
-            }
+                hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, p->cmax);
+                if (p->M > 0) {
+                    HASTY_PROF("nufft_gather_sorted", s);
+                    const int64_t n = p->M * nb;
+                    hipLaunchKernelGGL(gather_sorted_kernel<V>,
+                                       dim3(unsigned(std::min<int64_t>(65536, ceil_div(n, 256)))),
+                                       dim3(256), 0, s, static_cast<const V*>(cb),
+                                       static_cast<V*>(p->csort), static_cast<const int*>(p->sidx),
+                                       p->M, nb, p->cmax);
+                    HASTY_CHECK_HIP(hipGetLastError());
+                }
+                ga.subs = p->sub_spread;
+                {
+                    HASTY_PROF("nufft_spread", s);
+                    hipLaunchKernelGGL(reinterpret_cast<GridFn<V>>(kernel_ptr(p, 1)),
+                                       dim3(unsigned(p->nsub_spread), ngroups), dim3(kNufftThreads),
+                                       spread_lds_bytes(p, ga.group), s, ga);
+                    HASTY_CHECK_HIP(hipGetLastError());
+                }
+                if (p->nsplit > 0) {
+                    HASTY_PROF("nufft_spread_reduce", s);
+                    const int cells = p->T[0] * p->T[1] * p->T[2];
+                    const unsigned bx = unsigned(std::min<int64_t>(64, ceil_div(int64_t(nb) * cells, 256)));
+                    hipLaunchKernelGGL(reduce_split_tiles_kernel<V>, dim3(bx, unsigned(p->nsplit)), dim3(256),
+                                       0, s, fine, static_cast<const V*>(p->partial),
+                                       static_cast<const int4*>(p->split_info), nb, g);
+                    HASTY_CHECK_HIP(hipGetLastError());
+                }
             }
             DiagOp<V> none{};
             pruned_inverse<V>(pg, nb, fine, fb, bufA, bufB, p->sign, R(1), none, none, none, deapod,

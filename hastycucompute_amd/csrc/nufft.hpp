@@ -12,7 +12,8 @@
 //   sidx[s]          original index of sorted point s
 //   sl0[a][s]        first support cell along slot a (wrapped into [0, nf_a))
 //   sfrac[a][s]      u_a - l0_a in (w/2 - 1, w/2]  (fractional offset, R)
-//   visits[v]        8*s + (tile-neighbour bits) for spreading: tile-major
+//   sw[s][a][q]      phi(sfrac[a][s] - q), q < w   (window weights, from FP64)
+//   visits[v]        (8 s + tile-neighbour bits, l0) for spreading: tile-major
 //                    list of the points whose support touches each tile.
 #pragma once
 
@@ -23,8 +24,8 @@
 
 namespace hasty {
 
-constexpr int kNufftThreads = 256;
-constexpr int kMaxGroup = 4;        // transforms per spread / interp block
+constexpr int kMaxGroupSpread = 32;   // transforms per spreading block (2 lanes each)
+constexpr int kMaxGroupInterp = 4;    // transforms staged per interpolation block
 
 struct NufftPlan {
     int type = 1, dim = 3, sign = 1, ntransf = 1;
@@ -53,14 +54,22 @@ struct NufftPlan {
     int* sidx = nullptr;
     int* sl0 = nullptr;          // [3][M]
     void* sfrac = nullptr;       // [3][M] of R
-    int* visits = nullptr;
+    void* sw = nullptr;          // [M][3][w] of R: window weights per slot
+    void* csort = nullptr;       // [M][batch] sorted nonuniform data (spreading)
+    int64_t csort_cap = 0;
+    int cstride = 0;             // LDS words per transform in the spreading tile
+    int4* visits = nullptr;       // (8 s + neighbour bits, l0z, l0y, l0x) per tile visit
     int64_t nvisits = 0;
     int4* sub_interp = nullptr;
     int nsub_interp = 0;
     int4* sub_spread = nullptr;
     int nsub_spread = 0;
-    int* split_tiles = nullptr;
+    int4* split_info = nullptr;  // (tile, first partial slot, slot count, -) per split tile
     int nsplit = 0;
+    int nslots = 0;              // partial slots (one per subproblem of a split tile)
+    void* partial = nullptr;     // [nslots][batch][tile cells] partial spreading tiles
+    unsigned* cmax = nullptr;    // max |component| of the batch being spread (device)
+    int64_t partial_cap = 0;
     int64_t pts_cap = 0, visits_cap = 0;
     int64_t sub_interp_cap = 0, sub_spread_cap = 0, split_cap = 0;
 

@@ -183,3 +183,21 @@ def test_plan_misuse_raises():
     p.free()
     with pytest.raises(RuntimeError):
         p.setpts([torch.zeros(4, device="cuda")] * 3)
+
+
+@pytest.mark.parametrize("ntransf,kw", [(32, {}), (8, {"max_subproblem": 128}), (3, {"coil_group": 2}),
+                                        (32, {"tile": (4, 4, 4)})])
+def test_many_coil_kooshball_vs_kb_oracle(ntransf, kw):
+    """Radial sampling piles points onto the k-space centre: exercises split
+    tiles and the (transform, re/im) lane mapping of the spreading kernel."""
+    from hastycucompute_amd import synth
+    shape = (32, 32, 32)
+    coords = synth.kooshball(256, 64)
+    M = len(coords[0])
+    rng = np.random.default_rng(21)
+    f = crandn(rng, (ntransf,) + shape)
+    c = crandn(rng, (ntransf, M))
+    g1, g2 = _run(shape, coords, f, c, ntransf, **kw)
+    kb = KBPlan(shape, EPS).setpts(coords)
+    report(f"kooshball C={ntransf} {kw} type2 vs KB oracle", rel_l2(g2, kb.type2(f)), 2e-6)
+    report(f"kooshball C={ntransf} {kw} type1 vs KB oracle", rel_l2(g1, kb.type1(c)), 2e-6)

@@ -1,0 +1,74 @@
+"""CPU: host-side logic of the Python mirror (validation, enums, synthetic
+inputs) -- nothing here launches a kernel."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from hastycucompute_amd import fft as hf
+from hastycucompute_amd import synth
+
+
+def test_enums_mirror_reference():
+    # nufft.cppm:42-70, toeplitz.cppm:47-56
+    assert hf.nufft_type.FORWARD == hf.nufft_type.TYPE_2 == 2
+    assert hf.nufft_type.BACKWARD == hf.nufft_type.TYPE_1 == 1
+    assert hf.nufft_sign.DEFAULT_TYPE_1 == 1 and hf.nufft_sign.DEFAULT_TYPE_2 == -1
+    assert [int(t) for t in hf.ToeplitzMultType] == [0, 1, 2]
+    assert [int(t) for t in hf.ToeplitzAccumulateType] == [0, 1]
+
+
+def test_verify_coords_messages():
+    a = torch.zeros(4)
+    b = torch.zeros(5)
+    with pytest.raises(RuntimeError, match="x and y"):
+        hf.verify_coords([a, b])
+    with pytest.raises(RuntimeError, match="x,y"):
+        hf.verify_coords([a, a, b])
+    hf.verify_coords([a, a, a])
+
+
+def test_plan_rejects_type3_and_bad_dim_before_gpu():
+    with pytest.raises(RuntimeError):
+        hf.nufft_plan.make(hf.nufft_opts(nmodes=(8, 8, 8)), hf.nufft_type.TYPE_3)
+    with pytest.raises(RuntimeError):
+        hf.nufft_plan.make(hf.nufft_opts(nmodes=(8,)), hf.nufft_type.TYPE_1)
+    with pytest.raises(RuntimeError):
+        hf.nufft_plan.make(hf.nufft_opts(nmodes=(8, 8)), hf.nufft_type.TYPE_1, torch.float16)
+
+
+def test_cpu_tensors_are_refused():
+    with pytest.raises(RuntimeError, match="GPU"):
+        hf.toeplitz_multiply(torch.zeros((1, 4, 4, 4), dtype=torch.complex64),
+                             torch.zeros((8, 8, 8), dtype=torch.complex64))
+
+
+def test_kb_width_rule():
+    assert hf.kb_width(1e-5) == 6
+    assert hf.kb_width(1e-6) == 7
+    assert hf.kb_width(1e-9) == 10
+
+
+def test_kooshball_geometry():
+    kz, ky, kx = synth.kooshball(64, 32)
+    for k in (kz, ky, kx):
+        assert k.dtype == np.float32 and k.min() >= -np.pi - 1e-6 and k.max() <= np.pi + 1e-6
+    r = np.sqrt(kz.astype(np.float64) ** 2 + ky ** 2 + kx ** 2).reshape(64, 32)
+    expect = np.abs(-np.pi + 2 * np.pi * np.arange(32) / 32)
+    assert np.allclose(r, expect[None], atol=1e-5)
+    # golden-means directions: cos(theta) = frac(s * 0.4656)
+    s = 5
+    d = np.array([kz[s * 32], ky[s * 32], kx[s * 32]]) / -np.pi
+    assert abs(d[0] - math.fmod(s * synth.PHI1, 1.0)) < 1e-6
+
+
+def test_coil_maps_rss_normalised():
+    S = synth.coil_maps((12, 10, 8), 4)
+    rss = np.sqrt((np.abs(S) ** 2).sum(0))
+    assert np.allclose(rss, 1.0, atol=1e-5)
+
+
+def test_spiral_in_range():
+    ky, kx = synth.spiral2d(4, 256)
+    assert np.all(np.abs(ky) <= np.pi) and np.all(np.abs(kx) <= np.pi)

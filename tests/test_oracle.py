@@ -1,0 +1,164 @@
+"""CPU: pin the oracle against closed forms, invariants and brute force.
+
+The reference holds no golden vectors or known-answer tests for this path
+(its NUFFT is the external cufinufft and no reference test asserts a value),
+so every oracle function is pinned here by mathematics a plausible mistake
+would break:
+
+* ndft_type2 / ndft_type1 -- points on the Cartesian grid reduce to numpy's
+  FFT with fftshift (pins sign, centred modes and axis pairing); type 1 is the
+  adjoint of type 2; a point at k = 0 gives sum(f) / a constant image.
+* KBPlan -- against the NDFT (3-D, 2-D, batched), error shrinking with width.
+* toeplitz_kernel / toeplitz_multiply -- the Toeplitz embedding theorem
+  T(x) = (1/N^d) A^H A x against the NDFT pair; full Cartesian sampling gives
+  T = identity; a one-hot kernel reduces the product to a circular shift.
+* toeplitz_multiplication -- each diagonal mode against the explicit product.
+* conjugate_gradient -- exact convergence on a diagonal SPD operator with 3
+  distinct eigenvalues; agreement with numpy.linalg.solve on a dense SPD
+  matrix after n iterations.
+* max_eig -- numpy.linalg.eigvalsh.
+"""
+import numpy as np
+import pytest
+
+from oracle import toeplitz as ot
+from oracle.kb import KBPlan, kernel_beta, kernel_width
+from oracle.ndft import ndft_type1, ndft_type2, rel_l2
+from oracle.solvers import conjugate_gradient, max_eig, sense_normal, sense_normal_direct
+
+
+def crandn(rng, shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+def cartesian_coords(shape):
+    grids = np.meshgrid(*[2 * np.pi * (np.arange(n) - n // 2) / n for n in shape], indexing="ij")
+    return [g.ravel() for g in grids]
+
+
+@pytest.mark.parametrize("shape", [(8,), (6, 5), (4, 6, 5)])
+def test_ndft_cartesian_is_fft(shape):
+    rng = np.random.default_rng(0)
+    f = crandn(rng, shape)
+    coords = cartesian_coords(shape)
+    # c at Cartesian point m: sum_n f[n] e^{-2 pi i m.n / N} with centred m, n
+    ref = np.fft.fftshift(np.fft.fftn(np.fft.ifftshift(f))).ravel()
+    assert rel_l2(ndft_type2(f, coords), ref) < 1e-12
+    c = crandn(rng, int(np.prod(shape)))
+    ref1 = np.fft.fftshift(np.fft.ifftn(np.fft.ifftshift(c.reshape(shape)))) * np.prod(shape)
+    assert rel_l2(ndft_type1(c, coords, shape), ref1) < 1e-12
+
+
+def test_ndft_adjoint_and_origin():
+    rng = np.random.default_rng(1)
+    shape, M = (5, 7, 6), 300
+    coords = [rng.uniform(-np.pi, np.pi, M) for _ in range(3)]
+    f = crandn(rng, shape)
+    c = crandn(rng, M)
+    lhs = np.vdot(ndft_type2(f, coords), c)
+    rhs = np.vdot(f, ndft_type1(c, coords, shape))
+    assert abs(lhs - rhs) < 1e-10 * abs(lhs)
+    z = [np.zeros(1)] * 3
+    assert abs(ndft_type2(f, z)[0] - f.sum()) < 1e-12 * abs(f.sum())
+    assert np.allclose(ndft_type1(np.array([2 - 1j]), z, shape), 2 - 1j)
+
+
+def test_ndft_axis_pairing():
+    """coords[a] pairs with axis a: a point moving along k_0 only changes the
+    phase along axis 0."""
+    shape = (4, 1, 1)
+    f = np.zeros(shape, complex)
+    f[3, 0, 0] = 1.0                        # mode n0 = 3 - 2 = 1
+    c = ndft_type2(f, [np.array([0.5]), np.array([0.0]), np.array([0.0])])
+    assert abs(c[0] - np.exp(-0.5j)) < 1e-14
+
+
+@pytest.mark.parametrize("shape,M", [((16, 12, 10), 3000), ((40, 36), 3000)])
+def test_kb_vs_ndft(shape, M):
+    rng = np.random.default_rng(2)
+    coords = [rng.uniform(-np.pi, np.pi, M) for _ in shape]
+    f = crandn(rng, (2,) + shape)
+    c = crandn(rng, (2, M))
+    p = KBPlan(shape, 1e-5).setpts(coords)
+    assert rel_l2(p.type2(f), ndft_type2(f, coords)) < 2e-5
+    assert rel_l2(p.type1(c), ndft_type1(c, coords, shape)) < 2e-5
+
+
+def test_kb_error_decreases_with_width():
+    rng = np.random.default_rng(3)
+    shape, M = (12, 10, 14), 2000
+    coords = [rng.uniform(-np.pi, np.pi, M) for _ in shape]
+    f = crandn(rng, shape)
+    ref = ndft_type2(f, coords)
+    errs = [rel_l2(KBPlan(shape, tol).setpts(coords).type2(f), ref) for tol in (1e-3, 1e-5, 1e-7)]
+    assert errs[0] > errs[1] > errs[2]
+    assert kernel_width(1e-5) == 6 and abs(kernel_beta(6) - 13.8551) < 1e-3
+
+
+def test_toeplitz_embedding_theorem():
+    rng = np.random.default_rng(4)
+    shape, M, C = (6, 8, 5), 800, 3
+    coords = [rng.uniform(-np.pi, np.pi, M) for _ in shape]
+    x = crandn(rng, shape)
+    S = crandn(rng, (C,) + shape)
+    K = ot.toeplitz_kernel(coords, shape)
+    assert rel_l2(sense_normal(x, S, K), sense_normal_direct(x, S, coords)) < 1e-12
+
+
+def test_toeplitz_cartesian_identity_and_shift():
+    shape = (4, 6, 5)
+    K = ot.toeplitz_kernel(cartesian_coords(shape), shape)
+    rng = np.random.default_rng(5)
+    x = crandn(rng, (2,) + shape)
+    assert rel_l2(ot.toeplitz_multiply(x, K), x) < 1e-12
+    # a single lag m at kernel index m + N - 1 gives out[n] = x[n - m] (zero outside)
+    lag = np.zeros(tuple(2 * n for n in shape), complex)
+    lag[tuple(n - 1 + 1 for n in shape)] = 1.0            # lag m = (1, 1, 1)
+    out = ot.toeplitz_multiply(x[0], np.fft.fftn(lag))
+    ref = np.zeros(shape, complex)
+    ref[1:, 1:, 1:] = x[0][:-1, :-1, :-1]
+    assert rel_l2(out, ref) < 1e-12
+
+
+@pytest.mark.parametrize("io,i1,o1,acc", [(2, 1, 0, 0), (0, 2, 1, 1), (1, 0, 2, 0)])
+def test_toeplitz_multiplication_modes(io, i1, o1, acc):
+    rng = np.random.default_rng(6)
+    shape = (3, 4, 5)
+    K = np.fft.fftn(crandn(rng, tuple(2 * n for n in shape)))
+    inp = crandn(rng, (2,) + shape)
+    prev = crandn(rng, (2,) + shape)
+    m1 = crandn(rng, shape)
+    mult = {0: lambda a, b: a, 1: lambda a, b: a * b, 2: lambda a, b: a * np.conj(b)}
+    out = ot.toeplitz_multiplication(inp, prev, K, m1, None, io, i1, o1, 0, 0, bool(acc))
+    for b in range(2):
+        t = ot.toeplitz_multiply(mult[i1](inp[b], m1), K)
+        t = mult[io](mult[o1](t, m1), inp[b])
+        if acc:
+            t = t + prev[b]
+        assert rel_l2(out[b], t) < 1e-12
+
+
+def test_cg_exact_and_dense():
+    d = np.tile(np.array([1.0, 4.0, 9.0]), 20)
+    rng = np.random.default_rng(7)
+    b = crandn(rng, d.size)
+    x = conjugate_gradient(lambda v: d * v, b, np.zeros_like(b), 3)
+    assert rel_l2(x, b / d) < 1e-10
+    n = 12
+    Q = np.linalg.qr(crandn(rng, (n, n)))[0]
+    A = Q @ np.diag(np.linspace(1, 5, n)) @ Q.conj().T
+    b = crandn(rng, n)
+    hist = []
+    x = conjugate_gradient(lambda v: A @ v, b, np.zeros(n, complex), n, history=hist)
+    assert rel_l2(x, np.linalg.solve(A, b)) < 1e-9
+    assert hist[-1] < hist[0]
+
+
+def test_max_eig():
+    rng = np.random.default_rng(8)
+    n = 10
+    Q = np.linalg.qr(crandn(rng, (n, n)))[0]
+    ev = np.linspace(0.5, 3.0, n)
+    A = Q @ np.diag(ev) @ Q.conj().T
+    lam, _ = max_eig(lambda v: A @ v, crandn(rng, n), 200)
+    assert abs(lam - np.linalg.eigvalsh(A).max()) < 1e-6
