@@ -217,7 +217,7 @@ __global__ void fold_count_kernel(FoldArgs<CT> a, int* ccount, int* vcount) {
 
 template <class CT, class R>
 __global__ void fold_scatter_kernel(FoldArgs<CT> a, int* ccursor, int* vcursor, int* sidx,
-                                    int* sl0, R* sfrac, R* sw, int w, int ws, double beta,
+                                    int* sinv, int* sl0, R* sfrac, R* sw, int w, int ws, double beta,
                                     int4* visits) {
     const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (j >= a.M) return;
@@ -226,6 +226,7 @@ __global__ void fold_scatter_kernel(FoldArgs<CT> a, int* ccursor, int* vcursor, 
     fold_point(a, j, l0w, fr, t3);
     const int pos = atomicAdd(&ccursor[cell_key(a.g, l0w, t3)], 1);
     sidx[pos] = int(j);
+    sinv[j] = pos;
     const double ihw = 2.0 / double(w);
     for (int s = 0; s < 3; ++s) {
         sl0[int64_t(s) * a.M + pos] = l0w[s];
@@ -247,10 +248,6 @@ __global__ void fold_scatter_kernel(FoldArgs<CT> a, int* ccursor, int* vcursor, 
             }
 }
 
-__global__ void zero_u32_kernel(unsigned* p) {
-    if (threadIdx.x == 0) p[0] = 0u;
-}
-
 // toff[t] = first sorted point of tile t (t <= ntiles) from the cell offsets.
 __global__ void tile_offsets_kernel(const int* coff, int64_t cpt, int64_t ntiles, int M, int* toff) {
     const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -262,8 +259,8 @@ __global__ void tile_offsets_kernel(const int* coff, int64_t cpt, int64_t ntiles
 // Plan
 // ===========================================================================
 NufftPlan::~NufftPlan() {
-    void* ptrs[] = {deapod[0], deapod[1], deapod[2], fine, bufA, bufB, sidx, sl0, sfrac, sw, csort,
-                    visits, sub_interp, sub_spread, split_info, partial, cmax};
+    void* ptrs[] = {deapod[0], deapod[1], deapod[2], fine, bufA, bufB, sidx, sinv, sl0, sfrac, sw,
+                    csort, visits, sub_interp, sub_spread, split_info, partial};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -332,36 +329,6 @@ size_t interp_lds_bytes(const NufftPlan* p, int group) {
     return cells * size_t(group) * (p->f64 ? 16 : 8);
 }
 
-// 64-bit accumulators per transform of a spreading tile: 2 per cell, padded
-// so that the stride is 2 mod 32 (lane l = 2 transform + component lands on
-// accumulator l mod 32: consecutive lanes, consecutive 8-byte slots).
-int spread_cstride(const NufftPlan* p) {
-    const int cells = p->T[0] * p->T[1] * p->T[2];
-    int cs = 2 * cells;
-    cs += ((2 - cs) % 32 + 32) % 32;
-    return cs;
-}
-size_t spread_lds_bytes(const NufftPlan* p, int group) {
-    return size_t(spread_cstride(p)) * size_t(group) * 8;
-}
-
-// peak of the window (t = 0), raised to the number of non-degenerate slots
-double window_peak_pow(const NufftPlan* p) {
-    double peak = 0.0;
-    // phi(0) = I0(beta): power series
-    double q = 0.25 * p->beta * p->beta, term = 1.0, sum = 1.0;
-    for (int k = 1; k < 200; ++k) {
-        term *= q / (double(k) * double(k));
-        sum += term;
-        if (term < 1e-17 * sum) break;
-    }
-    peak = sum;
-    double r = 1.0;
-    for (int s = 0; s < 3; ++s)
-        if (p->nf[s] > 1) r *= peak;
-    return r * 1.0000001;
-}
-
 void set_lds_limit(void* fn, size_t bytes) {
     if (bytes > 64 * 1024)
         HASTY_CHECK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)));
@@ -415,21 +382,26 @@ NufftPlan* nufft_makeplan(int type, int dim, const int64_t* nmodes, int sign, in
         p->ntile[s] = int(p->nf[s] / p->T[s]);
     }
     p->ntiles = int64_t(p->ntile[0]) * p->ntile[1] * p->ntile[2];
-    const int maxg = type == 2 ? kMaxGroupInterp : kMaxGroupSpread;
-    int group = (opts && opts->coil_group > 0) ? opts->coil_group : (type == 2 ? 1 : maxg);
-    group = std::min(std::min(group, maxg), ntransf);
     const int method = opts ? opts->method : 3;
-    auto lds_of = [&](int g) {
-        return type == 2 ? interp_lds_bytes(p.get(), g) : spread_lds_bytes(p.get(), g);
-    };
-    // largest power of two (or ntransf) whose tile fits
-    while (group > 1 && lds_of(group) > kLdsMax) group = (group & (group - 1)) ? (group & (group - 1)) : group / 2;
-    p->gm = (method == 0 || method == 1) || lds_of(1) > kLdsMax;
-    p->group = group;
-    p->cstride = spread_cstride(p.get());
+    if (type == 2) {
+        int group = (opts && opts->coil_group > 0) ? opts->coil_group : 1;
+        group = std::min(std::min(group, kMaxGroupInterp), ntransf);
+        while (group > 1 && interp_lds_bytes(p.get(), group) > kLdsMax) --group;
+        p->group = group;
+        p->gm = (method == 0 || method == 1) || interp_lds_bytes(p.get(), 1) > kLdsMax;
+    } else {
+        // transforms per spreading block: a power of two covering the batch
+        // (register accumulators per lane), capped at 32 (16 for complex128)
+        const int want = (opts && opts->coil_group > 0) ? opts->coil_group : ntransf;
+        const int cap = f64 ? kMaxGroupSpread / 2 : kMaxGroupSpread;
+        int group = 1;
+        while (group < std::min(want, cap)) group *= 2;
+        p->group = group;
+        p->gm = (method == 0 || method == 1);
+    }
     p->max_batch = (opts && opts->max_batch > 0) ? std::min(opts->max_batch, ntransf) : ntransf;
     p->max_sub = (opts && opts->max_subproblem > 0) ? opts->max_subproblem : (type == 2 ? 8192 : 2048);
-    if (!p->gm) set_lds_limit(kernel_ptr(p.get(), type == 2 ? 0 : 1), lds_of(group));
+    if (!p->gm && type == 2) set_lds_limit(kernel_ptr(p.get(), 0), interp_lds_bytes(p.get(), p->group));
     // deapodisation factors 1/Phi(2 pi m / nf) per slot, centred modes
     for (int s = 0; s < 3; ++s) {
         const int64_t n = p->n[s];
@@ -476,10 +448,12 @@ void nufft_setpts_typed(NufftPlan* p, int64_t M, const void* c0, const void* c1,
     p->M = M;
     if (M > p->pts_cap) {
         dev_free(p->sidx, p->pts_cap * 4, p->device_bytes);
+        dev_free(p->sinv, p->pts_cap * 4, p->device_bytes);
         dev_free(p->sl0, p->pts_cap * 12, p->device_bytes);
         dev_free_void(p->sfrac, p->pts_cap * 3 * rsz, p->device_bytes);
         dev_free_void(p->sw, p->pts_cap * weight_stride(p->w) * rsz, p->device_bytes);
         p->sidx = dev_alloc<int>(M, p->device_bytes);
+        p->sinv = dev_alloc<int>(M, p->device_bytes);
         p->sl0 = dev_alloc<int>(3 * M, p->device_bytes);
         p->sfrac = dev_alloc<unsigned char>(3 * M * rsz, p->device_bytes);
         p->sw = dev_alloc<unsigned char>(M * weight_stride(p->w) * rsz, p->device_bytes);
@@ -581,7 +555,7 @@ void nufft_setpts_typed(NufftPlan* p, int64_t M, const void* c0, const void* c1,
             using RT = std::remove_pointer_t<decltype(rtag)>;
             auto fa = run_fold((CT*)nullptr);
             hipLaunchKernelGGL((fold_scatter_kernel<CT, RT>), dim3(blocks), dim3(256), 0, s, fa,
-                               coff, vbuf + nt, p->sidx, p->sl0,
+                               coff, vbuf + nt, p->sidx, p->sinv, p->sl0,
                                static_cast<RT*>(p->sfrac), static_cast<RT*>(p->sw), p->w,
                                weight_stride(p->w), p->beta, p->visits);
         };
@@ -607,6 +581,8 @@ void nufft_setpts(NufftPlan* p, int64_t M, const void* c0, const void* c1, const
 namespace {
 
 template <class V> using GridFn = void (*)(GridArgs<V>);
+template <class V>
+using SpreadFn = void (*)(GridArgs<V>, const int4*, const V*, const decltype(V::x)*);
 
 template <class V>
 void execute_impl(NufftPlan* p, V* c, V* f) {
@@ -627,13 +603,9 @@ void execute_impl(NufftPlan* p, V* c, V* f) {
     V* fine = static_cast<V*>(p->fine);
     V* bufA = static_cast<V*>(p->bufA);
     V* bufB = static_cast<V*>(p->bufB);
-    if (p->type == 1 && !p->gm) {
-        // the batch's max |component| (fixed-point scale of the spreading);
-        // allocated even for M == 0 since every tile block reads it
-        if (!p->cmax) p->cmax = dev_alloc<unsigned>(1, p->device_bytes);
-    }
     if (p->type == 1 && !p->gm && p->M > 0) {
-        const int64_t need = p->M * int64_t(p->max_batch);
+        // + padding: the spreading kernel reads a full group past the last point
+        const int64_t need = p->M * int64_t(p->max_batch) + kMaxGroupSpread;
         if (need > p->csort_cap) {
             dev_free_void(p->csort, p->csort_cap * int64_t(sizeof(V)), p->device_bytes);
             p->csort = dev_alloc<V>(need, p->device_bytes);
@@ -656,10 +628,8 @@ void execute_impl(NufftPlan* p, V* c, V* f) {
         ga.visits = p->visits;
         ga.M = p->M; ga.nbatch = nb; ga.group = std::min(p->group, nb); ga.w = p->w;
         ga.ws = weight_stride(p->w);
-        ga.cstride = p->cstride; ga.g = g;
+        ga.g = g;
         ga.partial = static_cast<V*>(p->partial);
-        ga.cmax = p->cmax;
-        ga.wmax3 = window_peak_pow(p);
         const unsigned ngroups = unsigned(ceil_div(nb, ga.group));
         if (p->type == 2) {
             DiagOp<V> none{};
@@ -688,23 +658,25 @@ void execute_impl(NufftPlan* p, V* c, V* f) {
                                        dim3(unsigned(ceil_div(p->M, 256))), dim3(256), 0, s, ga);
                 HASTY_CHECK_HIP(hipGetLastError());
             } else {
-                hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, p->cmax);
                 if (p->M > 0) {
-                    HASTY_PROF("nufft_gather_sorted", s);
-                    const int64_t n = p->M * nb;
-                    hipLaunchKernelGGL(gather_sorted_kernel<V>,
-                                       dim3(unsigned(std::min<int64_t>(65536, ceil_div(n, 256)))),
-                                       dim3(256), 0, s, static_cast<const V*>(cb),
-                                       static_cast<V*>(p->csort), static_cast<const int*>(p->sidx),
-                                       p->M, nb, p->cmax);
+                    HASTY_PROF("nufft_sort_values", s);
+                    hipLaunchKernelGGL(sort_values_kernel<V>, dim3(unsigned(ceil_div(p->M, 64))), dim3(256),
+                                       0, s, static_cast<const V*>(cb), static_cast<V*>(p->csort),
+                                       static_cast<const int*>(p->sinv), p->M, nb);
                     HASTY_CHECK_HIP(hipGetLastError());
                 }
                 ga.subs = p->sub_spread;
                 {
                     HASTY_PROF("nufft_spread", s);
-                    hipLaunchKernelGGL(reinterpret_cast<GridFn<V>>(kernel_ptr(p, 1)),
-                                       dim3(unsigned(p->nsub_spread), ngroups), dim3(kNufftThreads),
-                                       spread_lds_bytes(p, ga.group), s, ga);
+                    // one wave per 64-cell sub-block of the tile, up to 8 per block
+                    const int b0 = p->T[0] == 1 ? 1 : 4, b1 = p->T[0] == 1 ? 8 : 4, b2 = b1;
+                    const int64_t nsb = ceil_div(p->T[0], b0) * ceil_div(p->T[1], b1) * ceil_div(p->T[2], b2);
+                    const unsigned threads = unsigned(64 * std::min<int64_t>(8, nsb));
+                    const unsigned ng = unsigned(ceil_div(nb, p->group));
+                    void* fn = p->f64 ? nufft_spread_ptr<double2>(p->group) : nufft_spread_ptr<float2>(p->group);
+                    hipLaunchKernelGGL(reinterpret_cast<SpreadFn<V>>(fn), dim3(unsigned(p->nsub_spread), ng),
+                                       dim3(threads), 0, s, ga, static_cast<const int4*>(p->visits),
+                                       static_cast<const V*>(p->csort), static_cast<const R*>(p->sw));
                     HASTY_CHECK_HIP(hipGetLastError());
                 }
                 if (p->nsplit > 0) {

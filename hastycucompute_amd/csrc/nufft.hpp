@@ -10,6 +10,7 @@
 //   bufA  [B, n_z,  n_y,  nf_x]   after the y-axis pass (pruned)
 // Points are kept bin-sorted by fine-grid tile (T_z x T_y x T_x cells):
 //   sidx[s]          original index of sorted point s
+//   sinv[j]          sorted position of original point j
 //   sl0[a][s]        first support cell along slot a (wrapped into [0, nf_a))
 //   sfrac[a][s]      u_a - l0_a in (w/2 - 1, w/2]  (fractional offset, R)
 //   sw[s][a][q]      phi(sfrac[a][s] - q), q < w   (window weights, from FP64)
@@ -24,7 +25,7 @@
 
 namespace hasty {
 
-constexpr int kMaxGroupSpread = 32;   // transforms per spreading block (2 lanes each)
+constexpr int kMaxGroupSpread = 32;   // transforms per spreading block (registers per lane)
 constexpr int kMaxGroupInterp = 4;    // transforms staged per interpolation block
 
 struct NufftPlan {
@@ -52,12 +53,12 @@ struct NufftPlan {
     bool have_pts = false;
     bool gm = false;             // global-memory spread / interp kernels
     int* sidx = nullptr;
+    int* sinv = nullptr;
     int* sl0 = nullptr;          // [3][M]
     void* sfrac = nullptr;       // [3][M] of R
     void* sw = nullptr;          // [M][3][w] of R: window weights per slot
     void* csort = nullptr;       // [M][batch] sorted nonuniform data (spreading)
     int64_t csort_cap = 0;
-    int cstride = 0;             // LDS words per transform in the spreading tile
     int4* visits = nullptr;       // (8 s + neighbour bits, l0z, l0y, l0x) per tile visit
     int64_t nvisits = 0;
     int4* sub_interp = nullptr;
@@ -68,7 +69,6 @@ struct NufftPlan {
     int nsplit = 0;
     int nslots = 0;              // partial slots (one per subproblem of a split tile)
     void* partial = nullptr;     // [nslots][batch][tile cells] partial spreading tiles
-    unsigned* cmax = nullptr;    // max |component| of the batch being spread (device)
     int64_t partial_cap = 0;
     int64_t pts_cap = 0, visits_cap = 0;
     int64_t sub_interp_cap = 0, sub_spread_cap = 0, split_cap = 0;

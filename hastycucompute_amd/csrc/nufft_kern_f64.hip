@@ -8,7 +8,6 @@ template <> void* nufft_kernel_ptr<double2>(int which, int w) {
 #define HASTY_CASE(Wv)                                                                      \
     case Wv:                                                                                \
         return which == 0   ? reinterpret_cast<void*>(&interp_kernel<double2, Wv>)           \
-               : which == 1 ? reinterpret_cast<void*>(&spread_kernel<double2, Wv>)           \
                : which == 2 ? reinterpret_cast<void*>(&interp_gm_kernel<double2, Wv>)        \
                             : reinterpret_cast<void*>(&spread_gm_kernel<double2, Wv>);
         HASTY_CASE(2) HASTY_CASE(3) HASTY_CASE(4) HASTY_CASE(5) HASTY_CASE(6) HASTY_CASE(7)
@@ -17,6 +16,17 @@ template <> void* nufft_kernel_ptr<double2>(int which, int w) {
 #undef HASTY_CASE
     }
     throw Error(HASTY_ERR_INVALID, "complex128 plans support kernel widths 2..14 (tol >= 1e-13)");
+}
+
+template <> void* nufft_spread_ptr<double2>(int cg) {
+    switch (cg) {
+        case 1: return reinterpret_cast<void*>(&spread_kernel<double2, 1>);
+        case 2: return reinterpret_cast<void*>(&spread_kernel<double2, 2>);
+        case 4: return reinterpret_cast<void*>(&spread_kernel<double2, 4>);
+        case 8: return reinterpret_cast<void*>(&spread_kernel<double2, 8>);
+        case 16: return reinterpret_cast<void*>(&spread_kernel<double2, 16>);
+    }
+    throw Error(HASTY_ERR_INTERNAL, "no spreading kernel for this transform group");
 }
 
 }  // namespace hasty

@@ -41,13 +41,10 @@ struct GridArgs {
     const int* sl0;         // [3][M]
     const R* sw;            // [M][ws]: w weights per slot, slot-major, padded to ws
     const int4* subs;       // (tile, first, count, split)
-    const int4* visits;     // (8 s + tile-neighbour bits, l0z, l0y, l0x)
+    const int4* visits;     // (8 s + tile-neighbour bits, l0z, l0y, l0x) per tile visit
     int64_t M;
     int nbatch, group, w, ws;
-    int cstride;            // LDS words between transforms (spread)
     V* partial;             // [slot][nbatch][tile cells]: partial tiles of split tiles
-    const unsigned* cmax;   // bits of max(|Re c|, |Im c|) over the batch (spread)
-    double wmax3;           // (window peak)^d, bound of the weight product
     Geo g;
 };
 
@@ -140,185 +137,120 @@ __global__ void __launch_bounds__(kNufftThreads) interp_kernel(GridArgs<V> a) {
 }
 
 // ---------------------------------------------------------------------------
-// Spreading: one block per (tile subproblem, transform group of <= 32); the
-// block owns the tile's T_z x T_y x T_x cells of every transform in its group.
-//
-// Accumulation is exact 64-bit fixed point with integer LDS atomics (float
-// atomics on LDS run ~30x slower than integer ones on gfx950).  Every
-// contribution v = c w_z w_y w_x is scaled by 2^k, k chosen per batch so that
-// |v| 2^k < 2^51 (max |c| of the batch times the window peak cubed), and
-// rounded to an integer with the 1.5 * 2^52 magic-number addition in FP64;
-// a tile sums at most max_subproblem such terms, far below 2^63.  The sum is
-// therefore independent of the order of the atomics (deterministic), and its
-// absolute error is below (terms) * 2^-k-1.
-//
-// Each wave takes one visit at a time; its lanes are the (transform, re/im)
-// pairs, so the visit record, weights and clip ranges are wave-uniform
-// (scalar loads / branches) and the 64 atomics of one instruction hit 64
-// distinct words.  The next visit's record, weights and value are loaded
-// before the current visit's atomics are issued.  Split tiles write their
-// share to a partial slot.
+// Spreading, formulated as a gather: one block per (tile subproblem, group of
+// CG transforms).  Each wave owns a sub-block of 64 cells of the tile
+// (4 x 4 x 4 in 3-D, 1 x 8 x 8 in 2-D) and each lane one cell of it, with
+// the CG complex accumulators of its cell in registers (FP32 for complex64).
+// The wave walks the tile's visit list (points whose support touches the
+// tile), skips with scalar tests the points whose support misses its
+// sub-block, and every lane adds w_z w_y w_x c over the CG transforms of the
+// point.  No atomics and no LDS; each cell's sum is taken in visit order, so
+// the result is the same on every run of a plan.  A split tile writes its
+// share to a partial slot (summed by reduce_split_tiles_kernel).
 // ---------------------------------------------------------------------------
 // Broadcast lane 0's value to the wave as a scalar (the value is known to be
 // wave-uniform; this lets the compiler keep it in SGPRs and branch on it).
 __device__ inline int wave_uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
-__device__ inline float wave_uniform(float x) {
-    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
-}
-__device__ inline double wave_uniform(double x) {
-    const long long b = __double_as_longlong(x);
-    const int lo = __builtin_amdgcn_readfirstlane(int(b & 0xffffffffll));
-    const int hi = __builtin_amdgcn_readfirstlane(int(b >> 32));
-    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+
+// x mod n for x in (-n, 2n)
+__device__ inline int wrap_once(int x, int n) {
+    x += x < 0 ? n : 0;
+    return x - (x >= n ? n : 0);
 }
 
-// round(x) for |x| < 2^51 as a two's-complement 64-bit integer
-__device__ inline unsigned long long fixed_round(double x) {
-    const double magic = 6755399441055744.0;   // 1.5 * 2^52
-    return static_cast<unsigned long long>(__double_as_longlong(x + magic) -
-                                           __double_as_longlong(magic));
+// Do the arcs [a, a + la) and [b, b + lb) of the circle Z_n intersect?
+__device__ inline bool arcs_meet(int a, int la, int b, int lb, int n) {
+    return wrap_once(b - a, n) < la || wrap_once(a - b, n) < lb;
 }
 
-// Fixed-point scale 2^k with (peak contribution) * 2^k <= 2^51.
-__device__ inline double spread_scale(const unsigned* cmax_bits, double wmax3) {
-    if (cmax_bits == nullptr) return 1.0;
-    const double bound = double(__uint_as_float(*cmax_bits)) * wmax3;
-    if (!(bound > 0.0) || !(bound < 1e300)) return 1.0;
-    int e = 0;
-    (void)frexp(bound, &e);          // bound < 2^e
-    return ldexp(1.0, 51 - e);
-}
-
-template <class V, int W>
-__global__ void __launch_bounds__(kNufftThreads) spread_kernel(GridArgs<V> a) {
+// The read-only arrays come as separate restrict-qualified arguments: with no
+// aliasing stores, the wave-uniform loads (visit records, the point's CG
+// values) become scalar loads.  csort holds CG - 1 padding elements past the
+// last point, so the CG values of a point are read without bounds tests.
+template <class V, int CG>
+__global__ void __launch_bounds__(kNufftThreads)
+    spread_kernel(GridArgs<V> a, const int4* __restrict__ visits, const V* __restrict__ csort,
+                  const decltype(V::x)* __restrict__ sw) {
     using R = decltype(V::x);
-    using Acc = unsigned long long;
-    extern __shared__ unsigned char smem_raw[];
-    Acc* lds = reinterpret_cast<Acc*>(smem_raw);
     const int4 sub = a.subs[blockIdx.x];
     int t0[3];
     tile_origin(a.g, sub.x, t0);
-    const int Tz = a.g.T[0], Ty = a.g.T[1], Tx = a.g.T[2];
-    const int cells = Tz * Ty * Tx;
-    const int g0 = blockIdx.y * a.group;
-    const int ng = min(a.group, a.nbatch - g0);
-    const int CS = a.cstride;
-    for (int idx = threadIdx.x; idx < a.group * CS; idx += blockDim.x) lds[idx] = 0ull;
-    const double scale = spread_scale(a.cmax, a.wmax3);
-    __syncthreads();
-
+    const int T0 = a.g.T[0], T1 = a.g.T[1], T2 = a.g.T[2];
+    const int cells = T0 * T1 * T2;
+    const int n0 = int(a.g.nf[0]), n1 = int(a.g.nf[1]), n2 = int(a.g.nf[2]);
+    const int wa0 = a.g.wa[0], wa1 = a.g.wa[1], wa2 = a.g.wa[2];
+    const int B0 = T0 == 1 ? 1 : 4, B1 = T0 == 1 ? 8 : 4, B2 = T0 == 1 ? 8 : 4;
+    const int ns1 = (T1 + B1 - 1) / B1, ns2 = (T2 + B2 - 1) / B2;
+    const int nsb = ((T0 + B0 - 1) / B0) * ns1 * ns2;
+    const int g0 = blockIdx.y * CG;
+    const int ng = min(CG, a.nbatch - g0);
     const int lane = threadIdx.x & 63;
+    const int c2 = lane % B2, c1 = (lane / B2) % B1, c0 = lane / (B2 * B1);
     const int wave = wave_uniform(int(threadIdx.x >> 6));
     const int nwaves = blockDim.x >> 6;
-    const int gi = lane >> 1;
-    const int comp = lane & 1;
-    const bool active = gi < ng;
-    Acc* my = lds + (active ? gi : 0) * CS + comp;
-    const int end = sub.y + sub.z;
-    const int WS = a.ws;
-    int v = sub.y + wave;
-    int4 rec_n = make_int4(0, 0, 0, 0);
-    R w_n[3 * W];
-    R val_n = R(0);
-    auto fetch = [&](int vv) {
-        rec_n = a.visits[vv];
-        const int sn = rec_n.x >> 3;
-        const R* w = a.sw + int64_t(sn) * WS;
+    const int W = a.w;
+    const int64_t G = a.g.nf[0] * a.g.nf[1] * a.g.nf[2];
+    for (int sb = wave; sb < nsb; sb += nwaves) {
+        const int o2 = (sb % ns2) * B2, o1 = ((sb / ns2) % ns1) * B1, o0 = (sb / (ns2 * ns1)) * B0;
+        const int l0c = o0 + c0, l1c = o1 + c1, l2c = o2 + c2;     // lane's tile-local cell
+        const bool mine = l0c < T0 && l1c < T1 && l2c < T2;
+        const int G0 = t0[0] + o0, G1 = t0[1] + o1, G2 = t0[2] + o2;  // sub-block origin
+        const int x0 = G0 + c0, x1 = G1 + c1, x2 = G2 + c2;           // lane's cell (< nf + B)
+        R acc[2 * CG];
 #pragma unroll
-        for (int q = 0; q < 3 * W; ++q) w_n[q] = w[q];
-        val_n = R(0);
-        if (active) {
-            const V cv = a.csort[int64_t(sn) * a.nbatch + g0 + gi];
-            val_n = comp ? cv.y : cv.x;
-        }
-    };
-    if (v < end) fetch(v);
-    for (; v < end; v += nwaves) {
-        const int4 rec = make_int4(wave_uniform(rec_n.x), wave_uniform(rec_n.y), wave_uniform(rec_n.z),
-                                   wave_uniform(rec_n.w));
-        R wt[3 * W];
-#pragma unroll
-        for (int q = 0; q < 3 * W; ++q) wt[q] = wave_uniform(w_n[q]);
-        const double val = double(val_n) * scale;
-        if (v + nwaves < end) fetch(v + nwaves);
-        const int bits = rec.x & 7;
-        const int l0[3] = {rec.y, rec.z, rec.w};
-        int qlo[3], qhi[3], off[3];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            const int wa = a.g.wa[d];
-            if (a.g.ntile[d] == 1) {
-                qlo[d] = 0; qhi[d] = wa; off[d] = l0[d];
-            } else {
-                const int r = l0[d] % a.g.T[d];
-                const int split = min(wa, a.g.T[d] - r);
-                if ((bits >> (2 - d)) & 1) { qlo[d] = split; qhi[d] = wa; off[d] = -split; }
-                else { qlo[d] = 0; qhi[d] = split; off[d] = r; }
+        for (int k = 0; k < 2 * CG; ++k) acc[k] = R(0);
+        // 64 visit records at a time, one per lane: every lane tests its
+        // record against the sub-block, and the wave then takes the
+        // overlapping ones in visit order (ballot bits, low to high).
+        const int vend = sub.y + sub.z;
+        for (int vb = sub.y; vb < vend; vb += 64) {
+            const int v = vb + lane;
+            int4 rec = make_int4(0, 0, 0, 0);
+            bool hit = false;
+            if (v < vend) {
+                rec = visits[v];
+                hit = arcs_meet(rec.y, wa0, G0, B0, n0) && arcs_meet(rec.z, wa1, G1, B1, n1) &&
+                      arcs_meet(rec.w, wa2, G2, B2, n2);
             }
-        }
-        if (!active) continue;
-        double wxd[W];
+            unsigned long long mask = __ballot(hit);
+            while (mask) {
+                const int i = __ffsll(mask) - 1;
+                mask &= mask - 1;
+                const int s = __builtin_amdgcn_readlane(rec.x, i) >> 3;
+                const int p0 = __builtin_amdgcn_readlane(rec.y, i);
+                const int p1 = __builtin_amdgcn_readlane(rec.z, i);
+                const int p2 = __builtin_amdgcn_readlane(rec.w, i);
+                const int q0 = wrap_once(x0 - p0, n0), q1 = wrap_once(x1 - p1, n1),
+                          q2 = wrap_once(x2 - p2, n2);
+                const bool in = q0 < wa0 && q1 < wa1 && q2 < wa2;
+                // out-of-support lanes read weight 0 of the point (in bounds)
+                const R* wp = sw + int64_t(s) * a.ws;
+                const R w0 = wp[in ? q0 : 0], w1 = wp[W + (in ? q1 : 0)], w2 = wp[2 * W + (in ? q2 : 0)];
+                const V* cv = csort + int64_t(s) * a.nbatch + g0;
+                V c[CG];
 #pragma unroll
-        for (int q = 0; q < W; ++q) wxd[q] = double(wt[2 * W + q]);
-        const bool xwrap = a.g.ntile[2] == 1;   // single tile along x: indices wrap
+                for (int k = 0; k < CG; ++k) c[k] = cv[k];
+                const R wt = in ? w0 * w1 * w2 : R(0);
 #pragma unroll
-        for (int qz = 0; qz < W; ++qz) {
-            if (qz >= qlo[0] && qz < qhi[0]) {
-                const double vz = val * double(wt[qz]);
-                int lz = off[0] + qz; lz -= lz >= Tz ? Tz : 0;
-#pragma unroll
-                for (int qy = 0; qy < W; ++qy) {
-                    if (qy >= qlo[1] && qy < qhi[1]) {
-                        int ly = off[1] + qy; ly -= ly >= Ty ? Ty : 0;
-                        const double vzy = vz * double(wt[W + qy]);
-                        Acc* row = my + 2 * ((lz * Ty + ly) * Tx);
-                        if (!xwrap) {
-                            // off + qx < Tx for every qx in range: constant offsets
-                            Acc* rq = row + 2 * off[2];
-#pragma unroll
-                            for (int qx = 0; qx < W; ++qx)
-                                if (qx >= qlo[2] && qx < qhi[2])
-                                    atomicAdd(rq + 2 * qx, fixed_round(vzy * wxd[qx]));
-                        } else {
-#pragma unroll
-                            for (int qx = 0; qx < W; ++qx) {
-                                if (qx >= qlo[2] && qx < qhi[2]) {
-                                    int lx = off[2] + qx; lx -= lx >= Tx ? Tx : 0;
-                                    atomicAdd(row + 2 * lx, fixed_round(vzy * wxd[qx]));
-                                }
-                            }
-                        }
-                    }
+                for (int k = 0; k < CG; ++k) {
+                    acc[2 * k] += wt * c[k].x;
+                    acc[2 * k + 1] += wt * c[k].y;
                 }
             }
         }
-    }
-    __syncthreads();
-    const int64_t G = a.g.nf[0] * a.g.nf[1] * a.g.nf[2];
-    const double inv = 1.0 / scale;
-    auto value = [&](int g, int c) {
-        return cmake<V>(R(double(static_cast<long long>(lds[g * CS + 2 * c])) * inv),
-                        R(double(static_cast<long long>(lds[g * CS + 2 * c + 1])) * inv));
-    };
-    if (sub.w != 0) {
-        // split tile: this block's share goes to its partial slot (summed by
-        // reduce_split_tiles_kernel), contiguous per transform
-        V* dst = a.partial + (int64_t(sub.w - 1) * a.nbatch + g0) * cells;
-        for (int idx = threadIdx.x; idx < ng * cells; idx += blockDim.x) {
-            const int g = idx / cells;
-            dst[idx] = value(g, idx - g * cells);
+        if (!mine) continue;
+        const int cell = (l0c * T1 + l1c) * T2 + l2c;
+        if (sub.w != 0) {
+            V* dst = a.partial + (int64_t(sub.w - 1) * a.nbatch + g0) * cells + cell;
+#pragma unroll
+            for (int k = 0; k < CG; ++k)
+                if (k < ng) dst[int64_t(k) * cells] = cmake<V>(acc[2 * k], acc[2 * k + 1]);
+        } else {
+            V* dst = a.fine + int64_t(g0) * G + (int64_t(x0) * n1 + x1) * n2 + x2;
+#pragma unroll
+            for (int k = 0; k < CG; ++k)
+                if (k < ng) dst[int64_t(k) * G] = cmake<V>(acc[2 * k], acc[2 * k + 1]);
         }
-        return;
-    }
-    for (int idx = threadIdx.x; idx < ng * cells; idx += blockDim.x) {
-        const int g = idx / cells;
-        const int c = idx - g * cells;
-        const int lx = c % Tx;
-        const int ly = (c / Tx) % Ty;
-        const int lz = c / (Tx * Ty);
-        V* dst = a.fine + int64_t(g0 + g) * G +
-                 (int64_t(t0[0] + lz) * a.g.nf[1] + (t0[1] + ly)) * a.g.nf[2] + (t0[2] + lx);
-        *dst = value(g, c);
     }
 }
 
@@ -433,37 +365,28 @@ __global__ void __launch_bounds__(256) spread_gm_kernel(GridArgs<V> a) {
     }
 }
 
-// csort[s * B + t] = c[t * M + sidx[s]]: the nonuniform data in sorted,
-// transform-fastest order for the spreading lanes.
-// Also accumulates max(|Re|, |Im|) into *cmax_bits (non-negative floats
-// order like their bit patterns).
+// csort[sinv[j]][b] = c[b][j]: the nonuniform values in sorted point order,
+// transforms fastest.  A block reads 64 consecutive points of up to 32
+// transforms (coalesced rows), transposes them in LDS and writes each point's
+// transforms as one contiguous run.
 template <class V>
-__global__ void __launch_bounds__(256) gather_sorted_kernel(const V* __restrict__ c, V* __restrict__ csort,
-                                                            const int* __restrict__ sidx, int64_t M, int B,
-                                                            unsigned* cmax_bits) {
-    using R = decltype(V::x);
-    __shared__ float red[256];
-    const int64_t n = M * B;
-    float m = 0.f;
-    for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < n;
-         k += int64_t(gridDim.x) * blockDim.x) {
-        const int64_t s = k / B;
-        const int t = int(k - s * B);
-        const V v = c[int64_t(t) * M + sidx[s]];
-        csort[k] = v;
-        m = fmaxf(m, float(fmax(fabs(v.x), fabs(v.y))));
-    }
-    red[threadIdx.x] = m;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
+__global__ void __launch_bounds__(256) sort_values_kernel(const V* __restrict__ c, V* __restrict__ csort,
+                                                          const int* __restrict__ sinv, int64_t M, int B) {
+    __shared__ V tile[64][33];
+    const int64_t j0 = int64_t(blockIdx.x) * 64;
+    const int lane = threadIdx.x & 63, k0 = threadIdx.x >> 6;
+    const int64_t j = j0 + lane;
+    for (int b0 = 0; b0 < B; b0 += 32) {
+        const int nbc = min(32, B - b0);
+        if (j < M)
+            for (int k = k0; k < nbc; k += 4) tile[lane][k] = c[int64_t(b0 + k) * M + j];
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < 64 * nbc; idx += 256) {
+            const int pnt = idx / nbc, k = idx - pnt * nbc;
+            if (j0 + pnt < M) csort[int64_t(sinv[j0 + pnt]) * B + b0 + k] = tile[pnt][k];
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        // round up so the bound stays a bound after the float conversion
-        atomicMax(cmax_bits, __float_as_uint(red[0]) + (red[0] > 0.f ? 1u : 0u));
-    }
-    (void)sizeof(R);
 }
 
 template <class V>
@@ -496,5 +419,7 @@ __global__ void zero_tiles_kernel(V* fine, const int* tiles, int nbatch, Geo g) 
 
 // which: 0 interp (LDS), 1 spread (LDS), 2 interp (GM), 3 spread (GM)
 template <class V> void* nufft_kernel_ptr(int which, int w);
+// spreading kernel for CG transforms per block (1, 2, 4, 8, 16, 32 / 16 for complex128)
+template <class V> void* nufft_spread_ptr(int cg);
 
 }  // namespace hasty
