@@ -71,6 +71,7 @@ SIGNATURES = {
     "hasty_cg_status": (c_int, [c_vp, P_dbl, P_int, c_vp]),
     "hasty_scale_by_device": (c_int, [c_vp, c_i64, c_vp, c_int, c_vp]),
     "hasty_norm": (c_int, [c_vp, c_i64, c_vp, c_vp]),
+    "hasty_axpy": (c_int, [ctypes.c_double, ctypes.c_double, c_vp, c_vp, c_i64, c_vp]),
 }
 
 _lib = None

@@ -63,7 +63,29 @@ def build(jobs: int | None = None) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_cpp_test()
     return LIB
+
+
+CPP_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "test_corelib.cpp")
+CPP_TEST_BIN = os.path.join(OUT_DIR, "test_corelib")
+
+
+def build_cpp_test() -> str:
+    """Compile the C++ corelib-mirror test program (host code only, g++)
+    against the built library; it runs on a GPU from tests/test_gpu_cpp_api.py."""
+    os.makedirs(OUT_DIR, exist_ok=True)
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__",
+           "-I", os.path.join(ROOT, "include"), "-I", os.path.join(rocm, "include"),
+           CPP_TEST_SRC, "-L", HERE, "-lhasty_hip", "-L", os.path.join(rocm, "lib"), "-lamdhip64",
+           "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath," + os.path.join(rocm, "lib"),
+           "-o", CPP_TEST_BIN + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"C++ test build failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(CPP_TEST_BIN + ".tmp", CPP_TEST_BIN)
+    return CPP_TEST_BIN
 
 
 if __name__ == "__main__":

@@ -202,6 +202,19 @@ __global__ void scale_kernel(float2* x, int64_t n, const double* a, int invert) 
     }
 }
 
+// y <- y + a x (complex a in FP64, rounded to the c64 arithmetic once)
+__global__ void __launch_bounds__(kRedThreads) axpy_kernel(float2* y, const float2* x, int64_t n,
+                                                           float are, float aim) {
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * blockDim.x) {
+        const float2 v = x[i];
+        float2 o = y[i];
+        o.x += are * v.x - aim * v.y;
+        o.y += are * v.y + aim * v.x;
+        y[i] = o;
+    }
+}
+
 __global__ void zero_doubles_kernel(double* p, int n) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0.0;
 }
@@ -253,6 +266,17 @@ int hasty_scale_by_device(void* x, int64_t n, const double* a, int invert, void*
         HASTY_REQUIRE(n >= 0 && a, "invalid arguments to hasty_scale_by_device");
         hipLaunchKernelGGL(scale_kernel, dim3(red_blocks(n)), dim3(kRedThreads), 0,
                            static_cast<hipStream_t>(stream), static_cast<float2*>(x), n, a, invert);
+        HASTY_CHECK_HIP(hipGetLastError());
+    });
+}
+
+int hasty_axpy(double a_re, double a_im, const void* x, void* y, int64_t n, void* stream) {
+    return guarded([&] {
+        HASTY_REQUIRE(n >= 0 && (n == 0 || (x && y)), "invalid arguments to hasty_axpy");
+        if (n == 0) return;
+        hipLaunchKernelGGL(axpy_kernel, dim3(red_blocks(n)), dim3(kRedThreads), 0,
+                           static_cast<hipStream_t>(stream), static_cast<float2*>(y),
+                           static_cast<const float2*>(x), n, float(a_re), float(a_im));
         HASTY_CHECK_HIP(hipGetLastError());
     });
 }

@@ -1,0 +1,523 @@
+// corelib.hpp -- C++17 host mirror of HastyCuCompute's corelib operator API
+// for the non-Cartesian reconstruction path, over the C-ABI of hasty_hip.h.
+//
+// Names, template shapes, argument order and error behaviour follow the
+// reference so that call sites read the same:
+//   nufft_opts / nufft_plan<cuda_t,TT,DIM,NT>   cpp/corelib/lib/fft/nufft.cppm:42-310
+//   nufft_backward_cpu_over_cuda               nufft.cppm:316-382
+//   toeplitz_kernel / toeplitz_multiply        nufft.cppm:386-479
+//   fft::ToeplitzMultType, fft::ToeplitzAccumulateType,
+//   fft::transform_toeplitz_kernel, fft::toeplitz_multiplication
+//                                              cpp/corelib/lib/fft/toeplitz.cppm:47-93
+//   NORMAL_T_T1_OP<cuda_t,c64_t,DIM>          cpp/corelib/lib/mri/normal.cppm:43-93
+//   conjugate_gradient(sptr<Op>, max_iter, tol) cpp/corelib/lib/op/cg.cppm:122-184
+//   max_eig(A, x0, max_iter)                   cpp/corelib/lib/op/alg.cppm:13-30
+//
+// The reference's tensor<D,TT,RANK> wraps a torch tensor; here tensor<cuda_t,..>
+// is a small device buffer (hipMalloc'd and reference-counted, or a view of
+// caller memory) and tensor<cpu_t,..> a host vector, so the header needs only
+// the HIP runtime API and libhasty_hip.so.  Build a program with
+//   g++ -std=c++17 -D__HIP_PLATFORM_AMD__ -I<repo>/include -I/opt/rocm/include
+//       prog.cpp -L<repo>/hastycucompute_amd -lhasty_hip -L/opt/rocm/lib -lamdhip64
+// Every failure of the library is thrown as std::runtime_error carrying the
+// library's message (hasty_last_error).
+#ifndef HASTY_CORELIB_HPP
+#define HASTY_CORELIB_HPP
+
+#include <hip/hip_runtime_api.h>
+
+#include <array>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <functional>
+#include <memory>
+#include <numeric>
+#include <optional>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <type_traits>
+#include <vector>
+
+#include "../hasty_hip.h"
+
+namespace hasty {
+
+// ---- device tags and element types (util_typing in the reference) --------
+struct cuda_t {};
+struct cpu_t {};
+using f32_t = float;
+using f64_t = double;
+using c64_t = std::complex<float>;
+using c128_t = std::complex<double>;
+using i32 = int32_t;
+template <class T> using sptr = std::shared_ptr<T>;
+template <class T> using uptr = std::unique_ptr<T>;
+template <class T> using optrefw = std::optional<std::reference_wrapper<T>>;
+template <class T> using optcrefw = std::optional<std::reference_wrapper<const T>>;
+
+template <class TT> struct complex_of;
+template <> struct complex_of<f32_t> { using type = c64_t; };
+template <> struct complex_of<f64_t> { using type = c128_t; };
+template <class TT> using complex_t = typename complex_of<TT>::type;
+template <class TT> struct real_of { using type = TT; };
+template <class R> struct real_of<std::complex<R>> { using type = R; };
+template <class TT> using real_t = typename real_of<TT>::type;
+
+namespace detail {
+inline void check(int status) {
+    if (status != HASTY_OK) throw std::runtime_error(std::string("hasty: ") + hasty_last_error());
+}
+inline void check_hip(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+}  // namespace detail
+
+// ---- tensors ---------------------------------------------------------------
+template <class D, class TT, size_t RANK> class tensor;
+
+template <class TT, size_t RANK> class tensor<cuda_t, TT, RANK> {
+public:
+    tensor() = default;
+    // view of caller-owned device memory
+    tensor(TT* data, std::array<int64_t, RANK> shape) : _ptr(data), _shape(shape) {}
+
+    static tensor empty(std::array<int64_t, RANK> shape) {
+        tensor t;
+        t._shape = shape;
+        const size_t bytes = size_t(t.numel()) * sizeof(TT);
+        void* p = nullptr;
+        if (bytes) detail::check_hip(hipMalloc(&p, bytes), "hipMalloc");
+        t._owner = std::shared_ptr<void>(p, [](void* q) { if (q) (void)hipFree(q); });
+        t._ptr = static_cast<TT*>(p);
+        return t;
+    }
+    static tensor zeros(std::array<int64_t, RANK> shape) {
+        tensor t = empty(shape);
+        t.fill_zero();
+        return t;
+    }
+    static tensor from_host(const std::vector<TT>& h, std::array<int64_t, RANK> shape) {
+        tensor t = empty(shape);
+        if (int64_t(h.size()) != t.numel()) throw std::runtime_error("host data does not match shape");
+        detail::check_hip(hipMemcpy(t._ptr, h.data(), h.size() * sizeof(TT), hipMemcpyHostToDevice),
+                          "hipMemcpy H2D");
+        return t;
+    }
+    std::vector<TT> to_host() const {
+        std::vector<TT> h(static_cast<size_t>(numel()));
+        detail::check_hip(hipMemcpy(h.data(), _ptr, h.size() * sizeof(TT), hipMemcpyDeviceToHost),
+                          "hipMemcpy D2H");
+        return h;
+    }
+    tensor clone() const {
+        tensor t = empty(_shape);
+        detail::check_hip(hipMemcpy(t._ptr, _ptr, size_t(numel()) * sizeof(TT), hipMemcpyDeviceToDevice),
+                          "hipMemcpy D2D");
+        return t;
+    }
+    void fill_zero() {
+        if (numel()) detail::check_hip(hipMemset(_ptr, 0, size_t(numel()) * sizeof(TT)), "hipMemset");
+    }
+
+    template <size_t I> int64_t shape() const {
+        static_assert(I < RANK, "shape index out of range");
+        return _shape[I];
+    }
+    const std::array<int64_t, RANK>& shape_arr() const { return _shape; }
+    int64_t numel() const {
+        return std::accumulate(_shape.begin(), _shape.end(), int64_t(1), std::multiplies<int64_t>());
+    }
+    TT* mutable_data_ptr() { return _ptr; }
+    const TT* data_ptr() const { return _ptr; }
+    TT* unconsted_data_ptr() const { return _ptr; }
+
+private:
+    std::shared_ptr<void> _owner;
+    TT* _ptr = nullptr;
+    std::array<int64_t, RANK> _shape{};
+};
+
+template <class TT, size_t RANK> class tensor<cpu_t, TT, RANK> {
+public:
+    tensor() = default;
+    explicit tensor(std::array<int64_t, RANK> shape)
+        : _shape(shape), _data(static_cast<size_t>(std::accumulate(shape.begin(), shape.end(), int64_t(1),
+                                                                    std::multiplies<int64_t>()))) {}
+    template <size_t I> int64_t shape() const { return _shape[I]; }
+    const std::array<int64_t, RANK>& shape_arr() const { return _shape; }
+    int64_t numel() const { return int64_t(_data.size()); }
+    TT* mutable_data_ptr() { return _data.data(); }
+    const TT* data_ptr() const { return _data.data(); }
+    std::vector<TT>& vec() { return _data; }
+    const std::vector<TT>& vec() const { return _data; }
+
+private:
+    std::array<int64_t, RANK> _shape{};
+    std::vector<TT> _data;
+};
+
+template <class TT, size_t RANK>
+tensor<cuda_t, TT, RANK> make_empty_tensor(std::array<int64_t, RANK> shape) {
+    return tensor<cuda_t, TT, RANK>::empty(shape);
+}
+
+namespace detail {
+template <class TT, size_t RANK> void require_c64(const tensor<cuda_t, TT, RANK>&) {
+    static_assert(std::is_same_v<TT, c64_t>, "this operation is defined for c64_t tensors");
+}
+}  // namespace detail
+
+// ---- enums (nufft.cppm:42-70) ------------------------------------------------
+enum struct nufft_type { TYPE_1 = 1, TYPE_2 = 2, TYPE_3 = 3, FORWARD = TYPE_2, BACKWARD = TYPE_1 };
+enum struct nufft_sign { POS = 1, NEG = -1, DEFAULT_TYPE_1 = POS, DEFAULT_TYPE_2 = NEG };
+enum struct nufft_upsamp_cuda { UPSAMP_2_0, UPSAMP_1_25, DEFAULT };
+enum struct nufft_method_cuda { GM_NO_SORT, GM_SORT, SM, DEFAULT };
+
+// nufft.cppm:23-39 (same messages)
+template <class TT, size_t DIM>
+void verify_coords(const std::array<tensor<cuda_t, TT, 1>, DIM>& coords) {
+    if constexpr (DIM == 2) {
+        if (coords[0].template shape<0>() != coords[1].template shape<0>())
+            throw std::runtime_error("x and y coords have different lengths");
+    }
+    if constexpr (DIM == 3) {
+        if (coords[0].template shape<0>() != coords[1].template shape<0>() ||
+            coords[1].template shape<0>() != coords[2].template shape<0>() ||
+            coords[0].template shape<0>() != coords[2].template shape<0>())
+            throw std::runtime_error("x,y  x,z  or  y,z  coords have different lengths");
+    }
+}
+
+// ---- NUFFT options and plan (nufft.cppm:78-310) ------------------------------
+template <class D, class TT, size_t DIM> struct nufft_opts;
+
+template <class TT, size_t DIM> struct nufft_opts<cuda_t, TT, DIM> {
+    static_assert(DIM == 2 || DIM == 3, "this library supports 2-D and 3-D transforms");
+    std::array<int64_t, DIM> nmodes{};
+    int32_t ntransf = 1;
+    TT tol = TT(1e-6);
+    nufft_sign sign = nufft_sign::DEFAULT_TYPE_1;
+    nufft_upsamp_cuda upsamp = nufft_upsamp_cuda::DEFAULT;
+    nufft_method_cuda method = nufft_method_cuda::DEFAULT;
+    std::shared_ptr<hipStream_t> pstream;
+    int32_t device_idx = 0;
+    // tiling knobs of this library (0 = automatic)
+    std::array<int32_t, 3> tile{0, 0, 0};
+    int32_t coil_group = 0, max_batch = 0, max_subproblem = 0;
+
+    int32_t get_sign() const { return static_cast<int>(sign); }
+};
+
+template <class D, class TT, size_t DIM, nufft_type NT> struct nufft_plan;
+
+template <class TT, size_t DIM, nufft_type NT> struct nufft_plan<cuda_t, TT, DIM, NT> {
+    static_assert(std::is_same_v<TT, f32_t> || std::is_same_v<TT, f64_t>, "TT must be f32_t or f64_t");
+
+    static uptr<nufft_plan> make(const nufft_opts<cuda_t, TT, DIM>& opts) {
+        if (NT == nufft_type::TYPE_3) throw std::runtime_error("nufft type 3 is not supported");
+        uptr<nufft_plan> ret(new nufft_plan());
+        ret->_opts = opts;
+        hasty_nufft_opts o{};
+        switch (opts.method) {
+            case nufft_method_cuda::GM_NO_SORT: o.method = 0; break;
+            case nufft_method_cuda::GM_SORT: o.method = 1; break;
+            case nufft_method_cuda::SM: o.method = 2; break;
+            case nufft_method_cuda::DEFAULT: o.method = 3; break;
+            default: throw std::runtime_error("Invalid nufft_cuda_method");
+        }
+        switch (opts.upsamp) {
+            case nufft_upsamp_cuda::UPSAMP_2_0: o.upsamp = 2.0; break;
+            case nufft_upsamp_cuda::UPSAMP_1_25: o.upsamp = 1.25; break;
+            case nufft_upsamp_cuda::DEFAULT: o.upsamp = 0.0; break;
+            default: throw std::runtime_error("nufft_make_plan encountered invalid nufft_upsamp argument");
+        }
+        for (int a = 0; a < 3; ++a) o.tile[a] = opts.tile[size_t(a)];
+        o.coil_group = opts.coil_group;
+        o.max_batch = opts.max_batch;
+        o.max_subproblem = opts.max_subproblem;
+        void* stream = opts.pstream ? static_cast<void*>(*opts.pstream) : nullptr;
+        detail::check_hip(hipSetDevice(opts.device_idx), "hipSetDevice");
+        hasty_nufft_plan h = nullptr;
+        detail::check(hasty_nufft_makeplan(static_cast<int>(NT), int(DIM), opts.nmodes.data(),
+                                           opts.get_sign(), opts.ntransf, double(opts.tol),
+                                           std::is_same_v<TT, f64_t> ? HASTY_F64 : HASTY_F32, &o,
+                                           stream, &h));
+        ret->_plan = h;
+        return ret;
+    }
+
+    void setpts(const std::array<tensor<cuda_t, TT, 1>, DIM>& coords) {
+        verify_coords(coords);
+        const int64_t M = coords[0].template shape<0>();
+        const void* c[3] = {nullptr, nullptr, nullptr};
+        for (size_t a = 0; a < DIM; ++a) c[a] = coords[a].data_ptr();
+        detail::check(hasty_nufft_setpts(_plan, M, c[0], c[1], c[2]));
+        _M = M;
+    }
+
+    // type 1: [ntransf, M] -> [ntransf, N...]
+    template <nufft_type U = NT, std::enable_if_t<U == nufft_type::TYPE_1, int> = 0>
+    void execute(const tensor<cuda_t, complex_t<TT>, 2>& input,
+                 tensor<cuda_t, complex_t<TT>, DIM + 1>& output) const {
+        if (input.template shape<0>() != _opts.ntransf)
+            throw std::runtime_error("input first dimension must match ntransf");
+        check_modes(output.shape_arr());
+        if (input.template shape<1>() != _M) throw std::runtime_error("input length must match setpts");
+        detail::check(hasty_nufft_execute(_plan, input.unconsted_data_ptr(), output.mutable_data_ptr()));
+    }
+
+    // type 2: [ntransf, N...] -> [ntransf, M]
+    template <nufft_type U = NT, std::enable_if_t<U == nufft_type::TYPE_2, int> = 0>
+    void execute(const tensor<cuda_t, complex_t<TT>, DIM + 1>& input,
+                 tensor<cuda_t, complex_t<TT>, 2>& output) const {
+        check_modes(input.shape_arr());
+        if (output.template shape<0>() != _opts.ntransf || output.template shape<1>() != _M)
+            throw std::runtime_error("output must be [ntransf, M]");
+        detail::check(hasty_nufft_execute(_plan, output.mutable_data_ptr(), input.unconsted_data_ptr()));
+    }
+
+    const std::array<int64_t, DIM>& nmodes() const { return _opts.nmodes; }
+    int32_t ntransf() const { return _opts.ntransf; }
+    TT tol() const { return _opts.tol; }
+    nufft_opts<cuda_t, TT, DIM>& pa_opts() { return _opts; }
+    const nufft_opts<cuda_t, TT, DIM>& pa_opts() const { return _opts; }
+
+    void free() {
+        if (_plan != nullptr) {
+            const int st = hasty_nufft_destroy(_plan);
+            _plan = nullptr;
+            detail::check(st);
+        }
+    }
+    ~nufft_plan() {
+        if (_plan != nullptr) (void)hasty_nufft_destroy(_plan);
+    }
+    nufft_plan(const nufft_plan&) = delete;
+    nufft_plan& operator=(const nufft_plan&) = delete;
+
+private:
+    nufft_plan() = default;
+    template <size_t R> void check_modes(const std::array<int64_t, R>& s) const {
+        if (s[0] != _opts.ntransf) throw std::runtime_error("mode array first dimension must match ntransf");
+        for (size_t a = 0; a < DIM; ++a)
+            if (s[a + 1] != _opts.nmodes[a]) throw std::runtime_error("mode array shape must match nmodes");
+    }
+    nufft_opts<cuda_t, TT, DIM> _opts{};
+    hasty_nufft_plan _plan = nullptr;
+    int64_t _M = 0;
+};
+
+// nufft.cppm:316-382: the plain per-coil path (one plan of ntransf 1, one
+// host -> device -> host round trip per transform), kept for API parity.
+template <class TT, class TTC, size_t DIM>
+tensor<cpu_t, TT, DIM + 1> nufft_backward_cpu_over_cuda(const std::array<int64_t, DIM>& dimshape,
+                                                        const tensor<cpu_t, TT, 2>& input,
+                                                        const std::array<tensor<cuda_t, TTC, 1>, DIM>& coords) {
+    static_assert(std::is_same_v<TT, complex_t<TTC>>, "input must be complex of the coordinate type");
+    nufft_opts<cuda_t, TTC, DIM> options;
+    options.nmodes = dimshape;
+    options.ntransf = 1;
+    options.tol = std::is_same_v<TTC, f32_t> ? TTC(1e-6) : TTC(1e-14);
+    options.sign = nufft_sign::DEFAULT_TYPE_1;
+    const int64_t ntransf = input.template shape<0>();
+    const int64_t M = input.template shape<1>();
+    std::array<int64_t, DIM + 1> full{};
+    full[0] = ntransf;
+    for (size_t a = 0; a < DIM; ++a) full[a + 1] = dimshape[a];
+    tensor<cpu_t, TT, DIM + 1> output(full);
+    auto plan = nufft_plan<cuda_t, TTC, DIM, nufft_type::BACKWARD>::make(options);
+    plan->setpts(coords);
+    std::array<int64_t, DIM + 1> one = full;
+    one[0] = 1;
+    auto din = tensor<cuda_t, TT, 2>::empty({1, M});
+    auto dout = tensor<cuda_t, TT, DIM + 1>::empty(one);
+    const int64_t per = dout.numel();
+    for (int64_t i = 0; i < ntransf; ++i) {
+        detail::check_hip(hipMemcpy(din.mutable_data_ptr(), input.data_ptr() + i * M, size_t(M) * sizeof(TT),
+                                    hipMemcpyHostToDevice), "hipMemcpy H2D");
+        plan->execute(din, dout);
+        detail::check_hip(hipMemcpy(output.mutable_data_ptr() + i * per, dout.data_ptr(),
+                                    size_t(per) * sizeof(TT), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    }
+    return output;
+}
+
+// ---- Toeplitz (nufft.cppm:386-479) ---------------------------------------------
+// kernel [2N...] <- fftn(psf embedded at m + N - 1) / N^d; the PSF of weights
+// nudata [1, M] is taken with an FP64 type-1 NUFFT (the reference uses the
+// plan's own precision).
+template <class TT, size_t DIM>
+void toeplitz_kernel(const std::array<tensor<cuda_t, real_t<TT>, 1>, DIM>& coords,
+                     tensor<cuda_t, TT, DIM>& kernel, tensor<cuda_t, TT, 2>& nudata) {
+    detail::require_c64(kernel);
+    verify_coords(coords);
+    const int64_t M = coords[0].template shape<0>();
+    if (nudata.template shape<1>() != M)
+        throw std::runtime_error("shape<1>() of nudata didn't match length of coord vectors");
+    std::array<int64_t, DIM> n{};
+    for (size_t a = 0; a < DIM; ++a) n[a] = kernel.shape_arr()[a] / 2;
+    const void* c[3] = {nullptr, nullptr, nullptr};
+    for (size_t a = 0; a < DIM; ++a) c[a] = coords[a].data_ptr();
+    detail::check(hasty_toeplitz_kernel(int(DIM), n.data(), M, c[0], c[1], c[2], nudata.data_ptr(),
+                                        kernel.mutable_data_ptr(), 0.0, nullptr));
+}
+
+// inout[b] <- crop_{N-1}(ifftn(fftn_{2N}(inout[b]) * kernel))
+template <class TT, size_t DIM>
+void toeplitz_multiply(tensor<cuda_t, TT, DIM + 1>& inout, const tensor<cuda_t, TT, DIM>& kernel) {
+    detail::require_c64(kernel);
+    std::array<int64_t, DIM> n{};
+    for (size_t a = 0; a < DIM; ++a) {
+        n[a] = inout.shape_arr()[a + 1];
+        if (kernel.shape_arr()[a] != 2 * n[a]) throw std::runtime_error("kernel must have shape 2N");
+    }
+    detail::check(hasty_toeplitz_multiply(int(DIM), n.data(), inout.template shape<0>(),
+                                          inout.mutable_data_ptr(), kernel.data_ptr(), nullptr));
+}
+
+namespace fft {
+// toeplitz.cppm:47-56
+enum class ToeplitzMultType { NONE = 0, MULT = 1, MULT_CONJ = 2 };
+enum class ToeplitzAccumulateType { NONE = 0, ACCUMULATE = 1 };
+
+// toeplitz.cppm:510-609: the kernel stays in natural frequency order here,
+// so the transform only validates the shape.
+template <size_t DIM>
+void transform_toeplitz_kernel(tensor<cuda_t, c64_t, DIM>& kernel, bool clear_vkfft_plan = false) {
+    (void)clear_vkfft_plan;
+    std::array<int64_t, DIM> n{};
+    for (size_t a = 0; a < DIM; ++a) n[a] = kernel.shape_arr()[a] / 2;
+    detail::check(hasty_transform_toeplitz_kernel(int(DIM), n.data(), kernel.mutable_data_ptr(), nullptr));
+}
+
+// toeplitz.cppm:58-93 (3-D input [B, nx, ny] for DIM 2, [B, ...] for DIM 3)
+template <size_t R>
+void toeplitz_multiplication(const tensor<cuda_t, c64_t, R>& input, tensor<cuda_t, c64_t, R>& output,
+                             const tensor<cuda_t, c64_t, R - 1>& kernel,
+                             optrefw<tensor<cuda_t, c64_t, R - 1>> scratch,
+                             optcrefw<tensor<cuda_t, c64_t, R - 1>> mult1,
+                             optcrefw<tensor<cuda_t, c64_t, R - 1>> mult2,
+                             ToeplitzMultType input_output_mult_type, ToeplitzMultType input_mult1_type,
+                             ToeplitzMultType output_mult1_type, ToeplitzMultType input_mult2_type,
+                             ToeplitzMultType output_mult2_type, ToeplitzAccumulateType accumulate_type) {
+    constexpr size_t DIM = R - 1;
+    static_assert(DIM == 2 || DIM == 3, "toeplitz_multiplication supports 2-D and 3-D");
+    if (input.shape_arr() != output.shape_arr()) throw std::runtime_error("input and output shapes differ");
+    std::array<int64_t, DIM> n{};
+    for (size_t a = 0; a < DIM; ++a) {
+        n[a] = input.shape_arr()[a + 1];
+        if (kernel.shape_arr()[a] != 2 * n[a]) throw std::runtime_error("kernel must have shape 2N");
+        if (mult1 && mult1->get().shape_arr()[a] != n[a]) throw std::runtime_error("mult1 must have shape N");
+        if (mult2 && mult2->get().shape_arr()[a] != n[a]) throw std::runtime_error("mult2 must have shape N");
+    }
+    const int32_t types[5] = {int32_t(input_output_mult_type), int32_t(input_mult1_type),
+                              int32_t(output_mult1_type), int32_t(input_mult2_type),
+                              int32_t(output_mult2_type)};
+    void* scr = nullptr;
+    int64_t scr_bytes = 0;
+    if (scratch) {
+        scr = scratch->get().mutable_data_ptr();
+        scr_bytes = scratch->get().numel() * int64_t(sizeof(c64_t));
+    }
+    detail::check(hasty_toeplitz_multiplication(
+        int(DIM), n.data(), input.template shape<0>(), input.data_ptr(), output.mutable_data_ptr(),
+        kernel.data_ptr(), scr, scr_bytes, mult1 ? mult1->get().data_ptr() : nullptr,
+        mult2 ? mult2->get().data_ptr() : nullptr, types, int(accumulate_type), nullptr));
+}
+}  // namespace fft
+
+// ---- SENSE normal operator (normal.cppm:43-93) -----------------------------------
+// The reference holds the kernel and maps as cache_tensors (and uploads the
+// kernel on every apply); here both stay resident on the device.
+template <class D, class TT, size_t DIM> class NORMAL_T_T1_OP;
+
+template <size_t DIM> class NORMAL_T_T1_OP<cuda_t, c64_t, DIM> {
+public:
+    static constexpr const char* class_name = "NORMAL_T_T1_OP";
+    using device_type_t = cuda_t;
+    using input_tensor_type_t = c64_t;
+    using output_tensor_type_t = c64_t;
+    static constexpr size_t input_rank_t() { return DIM; }
+    static constexpr size_t output_rank_t() { return DIM; }
+
+    NORMAL_T_T1_OP(tensor<cuda_t, c64_t, DIM>&& kernel, tensor<cuda_t, c64_t, DIM + 1>&& stacked_diags,
+                   int coil_batch = 0)
+        : _kernel(std::move(kernel)), _smaps(std::move(stacked_diags)) {
+        for (size_t a = 0; a < DIM; ++a) {
+            _n[a] = _smaps.shape_arr()[a + 1];
+            if (_kernel.shape_arr()[a] != 2 * _n[a]) throw std::runtime_error("kernel must have shape 2N");
+        }
+        hasty_normal_op h = nullptr;
+        detail::check(hasty_normal_create(int(DIM), _n.data(), int(_smaps.template shape<0>()),
+                                          _smaps.data_ptr(), _kernel.data_ptr(), coil_batch, nullptr, &h));
+        _op = std::shared_ptr<hasty_normal_op_s>(h, [](hasty_normal_op q) { (void)hasty_normal_destroy(q); });
+    }
+
+    tensor<cuda_t, c64_t, DIM> operator()(const tensor<cuda_t, c64_t, DIM>& x) const {
+        auto out = tensor<cuda_t, c64_t, DIM>::empty(_n);
+        apply(x, out);
+        return out;
+    }
+    void apply(const tensor<cuda_t, c64_t, DIM>& x, tensor<cuda_t, c64_t, DIM>& out) const {
+        if (x.shape_arr() != _n || out.shape_arr() != _n) throw std::runtime_error("x must have shape N");
+        detail::check(hasty_normal_apply(_op.get(), x.data_ptr(), out.mutable_data_ptr()));
+    }
+
+private:
+    tensor<cuda_t, c64_t, DIM> _kernel;
+    tensor<cuda_t, c64_t, DIM + 1> _smaps;
+    std::array<int64_t, DIM> _n{};
+    std::shared_ptr<hasty_normal_op_s> _op;
+};
+
+// ---- CG (cg.cppm:122-184) and power iteration (alg.cppm:13-30) -----------------
+// Returns cg(x, b) -> x.  The step arithmetic runs in the library's FP64
+// reduction kernels with every scalar on the device; the residual is read
+// on the host once per iteration, as the reference does (cg.cppm:176).
+template <class Op> auto conjugate_gradient(sptr<Op> A, i32 max_iter = 0, double tol = 1e-6) {
+    constexpr size_t R = Op::input_rank_t();
+    using T = tensor<cuda_t, c64_t, R>;
+    return [A = std::move(A), max_iter, tol](T& x, const T& b) -> T& {
+        const int64_t n = x.numel();
+        auto state = tensor<cuda_t, double, 1>::zeros({HASTY_CG_STATE_DOUBLES});
+        T r = b.clone();
+        {
+            T Ax = (*A)(x);
+            detail::check(hasty_axpy(-1.0, 0.0, Ax.data_ptr(), r.mutable_data_ptr(), n, nullptr));
+        }
+        T p = T::empty(x.shape_arr());
+        detail::check(hasty_cg_init(r.mutable_data_ptr(), p.mutable_data_ptr(), n, tol,
+                                    state.mutable_data_ptr(), nullptr));
+        double resid = 0.0;
+        int done = 0;
+        detail::check(hasty_cg_status(state.data_ptr(), &resid, &done, nullptr));
+        T Ap = T::empty(x.shape_arr());
+        for (i32 iter = 0; iter < max_iter && !done; ++iter) {
+            A->apply(p, Ap);
+            detail::check(hasty_cg_step(x.mutable_data_ptr(), r.mutable_data_ptr(), p.mutable_data_ptr(),
+                                        Ap.data_ptr(), n, state.mutable_data_ptr(), nullptr));
+            detail::check(hasty_cg_status(state.data_ptr(), &resid, &done, nullptr));
+        }
+        return x;
+    };
+}
+
+template <class Op, class T>
+std::tuple<float, T> max_eig(const Op& A, T&& x0, i32 max_iter = 10) {
+    auto nrm = tensor<cuda_t, double, 1>::zeros({2});
+    T x = std::move(x0);
+    for (i32 i = 0; i < max_iter; ++i) {
+        T y = A(x);
+        detail::check(hasty_norm(y.data_ptr(), y.numel(), nrm.mutable_data_ptr(), nullptr));
+        detail::check(hasty_scale_by_device(y.mutable_data_ptr(), y.numel(), nrm.data_ptr(), 1, nullptr));
+        x = std::move(y);
+    }
+    const double lam = nrm.to_host()[0];
+    return {float(lam), std::move(x)};
+}
+
+}  // namespace hasty
+
+#endif  // HASTY_CORELIB_HPP

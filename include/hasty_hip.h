@@ -153,6 +153,9 @@ int hasty_cg_status(const double* state, double* resid, int* done, void* stream)
 int hasty_scale_by_device(void* x, int64_t n, const double* a, int invert, void* stream);
 /* out[0] = ||x||_2 (FP64, device) */
 int hasty_norm(const void* x, int64_t n, double* out, void* stream);
+/* y <- y + a x over n c64 elements (the b - A x and x + alpha p updates that
+ * cg.cppm:154-157 writes as tensor expressions) */
+int hasty_axpy(double a_re, double a_im, const void* x, void* y, int64_t n, void* stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,248 @@
+// GPU test of the C++ corelib mirror (include/hasty/corelib.hpp): the calls
+// read like the reference's own (cpp/corelib/interface/test_interface.cpp),
+// and results are checked against a direct double-precision NDFT written
+// here from the definition.  Prints one line per check; exit 0 = all passed.
+#include <hasty/corelib.hpp>
+
+#include <cstdio>
+#include <random>
+
+using namespace hasty;
+
+namespace {
+
+int failures = 0;
+
+void report(const char* what, double err, double tol) {
+    const bool ok = err <= tol;
+    std::printf("[cpp] %-46s rel_l2 = %.3e (tol %.1e) %s\n", what, err, tol, ok ? "ok" : "FAIL");
+    std::fflush(stdout);
+    if (!ok) ++failures;
+}
+
+template <class A, class B> double rel_l2(const std::vector<A>& a, const std::vector<B>& b) {
+    double num = 0, den = 0;
+    for (size_t i = 0; i < a.size(); ++i) {
+        const std::complex<double> x(a[i].real(), a[i].imag()), y(b[i].real(), b[i].imag());
+        num += std::norm(x - y);
+        den += std::norm(y);
+    }
+    return std::sqrt(num / den);
+}
+
+struct Problem {
+    std::array<int64_t, 3> n{12, 16, 10};
+    int64_t M = 3000;
+    int C = 3;
+    std::array<std::vector<float>, 3> k;
+    std::vector<c64_t> x, S;
+};
+
+Problem make_problem() {
+    Problem p;
+    std::mt19937_64 g(7);
+    std::uniform_real_distribution<float> u(-3.14159f, 3.14159f);
+    std::normal_distribution<float> nd(0.f, 1.f);
+    for (auto& ka : p.k) {
+        ka.resize(size_t(p.M));
+        for (auto& v : ka) v = u(g);
+    }
+    const int64_t I = p.n[0] * p.n[1] * p.n[2];
+    p.x.resize(size_t(I));
+    for (auto& v : p.x) v = c64_t(nd(g), nd(g));
+    p.S.resize(size_t(p.C * I));
+    for (auto& v : p.S) v = c64_t(nd(g), nd(g));
+    return p;
+}
+
+// c_j = sum_n f[n] exp(sign i k_j . n), centred modes, axis a <-> k[a]
+std::vector<std::complex<double>> ndft2(const Problem& p, const std::vector<c64_t>& f, int sign) {
+    std::vector<std::complex<double>> c(size_t(p.M));
+    for (int64_t j = 0; j < p.M; ++j) {
+        std::complex<double> acc = 0;
+        for (int64_t a = 0; a < p.n[0]; ++a) {
+            const double pa = double(p.k[0][size_t(j)]) * double(a - p.n[0] / 2);
+            for (int64_t b = 0; b < p.n[1]; ++b) {
+                const double pb = pa + double(p.k[1][size_t(j)]) * double(b - p.n[1] / 2);
+                for (int64_t q = 0; q < p.n[2]; ++q) {
+                    const double ph = pb + double(p.k[2][size_t(j)]) * double(q - p.n[2] / 2);
+                    const c64_t v = f[size_t((a * p.n[1] + b) * p.n[2] + q)];
+                    acc += std::complex<double>(v.real(), v.imag()) * std::polar(1.0, sign * ph);
+                }
+            }
+        }
+        c[size_t(j)] = acc;
+    }
+    return c;
+}
+
+std::vector<std::complex<double>> ndft1(const Problem& p, const std::vector<std::complex<double>>& c,
+                                        int sign) {
+    const int64_t I = p.n[0] * p.n[1] * p.n[2];
+    std::vector<std::complex<double>> f(size_t(I), 0.0);
+    for (int64_t a = 0; a < p.n[0]; ++a)
+        for (int64_t b = 0; b < p.n[1]; ++b)
+            for (int64_t q = 0; q < p.n[2]; ++q) {
+                std::complex<double> acc = 0;
+                for (int64_t j = 0; j < p.M; ++j) {
+                    const double ph = double(p.k[0][size_t(j)]) * double(a - p.n[0] / 2) +
+                                      double(p.k[1][size_t(j)]) * double(b - p.n[1] / 2) +
+                                      double(p.k[2][size_t(j)]) * double(q - p.n[2] / 2);
+                    acc += c[size_t(j)] * std::polar(1.0, sign * ph);
+                }
+                f[size_t((a * p.n[1] + b) * p.n[2] + q)] = acc;
+            }
+    return f;
+}
+
+std::array<tensor<cuda_t, f32_t, 1>, 3> device_coords(const Problem& p) {
+    return {tensor<cuda_t, f32_t, 1>::from_host(p.k[0], {p.M}),
+            tensor<cuda_t, f32_t, 1>::from_host(p.k[1], {p.M}),
+            tensor<cuda_t, f32_t, 1>::from_host(p.k[2], {p.M})};
+}
+
+template <class F> void expect_throw(const char* what, const char* needle, F&& f) {
+    try {
+        f();
+    } catch (const std::runtime_error& e) {
+        const bool ok = std::string(e.what()).find(needle) != std::string::npos;
+        std::printf("[cpp] %-46s threw \"%s\" %s\n", what, e.what(), ok ? "ok" : "FAIL");
+        if (!ok) ++failures;
+        return;
+    }
+    std::printf("[cpp] %-46s did not throw FAIL\n", what);
+    ++failures;
+}
+
+}  // namespace
+
+int main() {
+    const Problem p = make_problem();
+    const int64_t I = p.n[0] * p.n[1] * p.n[2];
+    auto coords = device_coords(p);
+
+    // type 2 over C transforms of S_c x
+    std::vector<c64_t> Sx(size_t(p.C * I));
+    for (int c = 0; c < p.C; ++c)
+        for (int64_t i = 0; i < I; ++i) Sx[size_t(c * I + i)] = p.S[size_t(c * I + i)] * p.x[size_t(i)];
+    nufft_opts<cuda_t, f32_t, 3> o2;
+    o2.nmodes = p.n;
+    o2.ntransf = p.C;
+    o2.tol = 1e-5f;
+    o2.sign = nufft_sign::DEFAULT_TYPE_2;
+    auto plan2 = nufft_plan<cuda_t, f32_t, 3, nufft_type::FORWARD>::make(o2);
+    plan2->setpts(coords);
+    auto dimg = tensor<cuda_t, c64_t, 4>::from_host(Sx, {p.C, p.n[0], p.n[1], p.n[2]});
+    auto dk = tensor<cuda_t, c64_t, 2>::empty({p.C, p.M});
+    plan2->execute(dimg, dk);
+    const auto hk = dk.to_host();
+    std::vector<std::vector<std::complex<double>>> ref2(size_t(p.C));
+    for (int c = 0; c < p.C; ++c) {
+        std::vector<c64_t> f(Sx.begin() + c * I, Sx.begin() + (c + 1) * I);
+        ref2[size_t(c)] = ndft2(p, f, -1);
+        std::vector<c64_t> got(hk.begin() + c * p.M, hk.begin() + (c + 1) * p.M);
+        report(("nufft_plan<TYPE_2> coil " + std::to_string(c) + " vs NDFT").c_str(),
+               rel_l2(got, ref2[size_t(c)]), 2e-5);
+    }
+
+    // type 1 of the same data
+    nufft_opts<cuda_t, f32_t, 3> o1 = o2;
+    o1.sign = nufft_sign::DEFAULT_TYPE_1;
+    auto plan1 = nufft_plan<cuda_t, f32_t, 3, nufft_type::BACKWARD>::make(o1);
+    plan1->setpts(coords);
+    auto dadj = tensor<cuda_t, c64_t, 4>::empty({p.C, p.n[0], p.n[1], p.n[2]});
+    plan1->execute(dk, dadj);
+    const auto hadj = dadj.to_host();
+    {
+        std::vector<std::complex<double>> c0(hk.begin(), hk.begin() + p.M);
+        const auto ref1 = ndft1(p, c0, +1);
+        std::vector<c64_t> got(hadj.begin(), hadj.begin() + I);
+        report("nufft_plan<TYPE_1> coil 0 vs NDFT", rel_l2(got, ref1), 2e-5);
+    }
+
+    // the per-coil host path gives the batched result
+    {
+        tensor<cpu_t, c64_t, 2> hin({p.C, p.M});
+        hin.vec() = hk;
+        auto hout = nufft_backward_cpu_over_cuda<c64_t, f32_t, 3>(p.n, hin, coords);
+        report("nufft_backward_cpu_over_cuda vs batched plan", rel_l2(hout.vec(), hadj), 2e-5);
+    }
+
+    // Toeplitz kernel + SENSE normal operator vs NDFT: sum_c conj(S_c) (1/N^3) A^H A (S_c x)
+    auto kernel = tensor<cuda_t, c64_t, 3>::empty({2 * p.n[0], 2 * p.n[1], 2 * p.n[2]});
+    auto ones = tensor<cuda_t, c64_t, 2>::from_host(std::vector<c64_t>(size_t(p.M), c64_t(1.f, 0.f)), {1, p.M});
+    toeplitz_kernel<c64_t, 3>(coords, kernel, ones);
+    std::vector<std::complex<double>> ref_n(size_t(I), 0.0);
+    for (int c = 0; c < p.C; ++c) {
+        const auto back = ndft1(p, ref2[size_t(c)], +1);
+        for (int64_t i = 0; i < I; ++i) {
+            const c64_t s = p.S[size_t(c * I + i)];
+            ref_n[size_t(i)] += std::conj(std::complex<double>(s.real(), s.imag())) * back[size_t(i)] / double(I);
+        }
+    }
+    auto smaps = tensor<cuda_t, c64_t, 4>::from_host(p.S, {p.C, p.n[0], p.n[1], p.n[2]});
+    auto op = std::make_shared<NORMAL_T_T1_OP<cuda_t, c64_t, 3>>(kernel.clone(), std::move(smaps));
+    auto dx = tensor<cuda_t, c64_t, 3>::from_host(p.x, p.n);
+    const auto hn = (*op)(dx).to_host();
+    report("NORMAL_T_T1_OP vs NDFT normal operator", rel_l2(hn, ref_n), 2e-5);
+
+    // toeplitz_multiplication with the reference's NORMAL_T_T1_OP modes, then sum
+    {
+        auto in = tensor<cuda_t, c64_t, 4>::from_host(p.S, {p.C, p.n[0], p.n[1], p.n[2]});
+        auto out = tensor<cuda_t, c64_t, 4>::empty(in.shape_arr());
+        fft::toeplitz_multiplication<4>(in, out, kernel, std::nullopt, std::cref(dx), std::nullopt,
+                                        fft::ToeplitzMultType::MULT_CONJ, fft::ToeplitzMultType::MULT,
+                                        fft::ToeplitzMultType::NONE, fft::ToeplitzMultType::NONE,
+                                        fft::ToeplitzMultType::NONE, fft::ToeplitzAccumulateType::NONE);
+        const auto h = out.to_host();
+        std::vector<c64_t> sum(size_t(I), c64_t(0.f, 0.f));
+        for (int c = 0; c < p.C; ++c)
+            for (int64_t i = 0; i < I; ++i) sum[size_t(i)] += h[size_t(c * I + i)];
+        report("toeplitz_multiplication + sum vs NDFT", rel_l2(sum, ref_n), 2e-5);
+    }
+
+    // toeplitz_multiply on S_0 x vs (1/N^3) A^H A (S_0 x)
+    {
+        std::vector<c64_t> s0(Sx.begin(), Sx.begin() + I);
+        auto t = tensor<cuda_t, c64_t, 4>::from_host(s0, {1, p.n[0], p.n[1], p.n[2]});
+        toeplitz_multiply<c64_t, 3>(t, kernel);
+        auto back = ndft1(p, ref2[0], +1);
+        for (auto& v : back) v /= double(I);
+        report("toeplitz_multiply vs NDFT", rel_l2(t.to_host(), back), 2e-5);
+    }
+
+    // CG on the normal equations: 30 iterations reduce the residual by 1e3
+    {
+        auto b = (*op)(dx);
+        auto x = tensor<cuda_t, c64_t, 3>::zeros(p.n);
+        auto cg = conjugate_gradient(op, 30, 0.0);
+        cg(x, b);
+        const auto Ax = (*op)(x).to_host();
+        const auto hb = b.to_host();
+        report("conjugate_gradient: ||A x - b|| / ||b||", rel_l2(Ax, hb), 1e-3);
+        auto [lam, v] = max_eig(*op, dx.clone(), 20);
+        std::printf("[cpp] max_eig estimate %.6f\n", double(lam));
+        if (!(lam > 0.f) || !std::isfinite(lam)) ++failures;
+    }
+
+    // error behaviour mirrors the reference's messages
+    expect_throw("verify_coords length mismatch", "coords have different lengths", [&] {
+        std::array<tensor<cuda_t, f32_t, 1>, 3> bad = {coords[0], coords[1],
+                                                       tensor<cuda_t, f32_t, 1>::empty({p.M - 1})};
+        plan2->setpts(bad);
+    });
+    expect_throw("execute with wrong ntransf", "input first dimension must match ntransf", [&] {
+        auto small = tensor<cuda_t, c64_t, 2>::empty({1, p.M});
+        plan1->execute(small, dadj);
+    });
+    expect_throw("library error surfaces as runtime_error", "hasty", [&] {
+        nufft_opts<cuda_t, f32_t, 3> bad = o2;
+        bad.tol = 0.f;
+        (void)nufft_plan<cuda_t, f32_t, 3, nufft_type::FORWARD>::make(bad);
+    });
+
+    plan1->free();
+    plan2->free();
+    std::printf("[cpp] %s (%d failure%s)\n", failures ? "FAILED" : "ALL OK", failures, failures == 1 ? "" : "s");
+    return failures ? 1 : 0;
+}
