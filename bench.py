@@ -104,6 +104,75 @@ def cpu_baseline(N, C, smaps_np, x_np, kernel_np, sample_coils):
             "seconds_per_apply": per_apply}
 
 
+# Kernels whose HBM traffic is measured with rocprofv3 --pmc, by the symbol
+# substring that identifies their launches (the Toeplitz passes are the
+# two-level FFT kernel in its column-PSF mode, unique to P3), plus the
+# calibration kernel: torch.neg over 1 GiB reads and writes exactly 2^30 B.
+PMC_KERNELS = {
+    "toep_P3_z_psf": "fft2_pass_kernel<HIP_vector_type<float, 2u>, 16, 16, -1, 1>",
+    "nufft_spread": "spread_kernel<HIP_vector_type<float, 2u>, 32>",
+    "nufft_interp": "interp_kernel<HIP_vector_type<float, 2u>, 6>",
+}
+CALIB_SUBSTR = "neg_kernel"
+CALIB_BYTES = 1 << 30
+
+
+def _pmc_rows(d):
+    """(kernel name, counter value) rows of every counter_collection CSV under d."""
+    import csv
+    import glob
+    rows = []
+    for f in glob.glob(os.path.join(d, "**", "*counter_collection*.csv"), recursive=True):
+        with open(f) as fh:
+            for r in csv.DictReader(fh):
+                name = r.get("Kernel_Name") or r.get("Kernel-Name") or ""
+                val = r.get("Counter_Value") or r.get("Counter-Value")
+                if val is not None:
+                    rows.append((name, float(val)))
+    return rows
+
+
+def reduce_pmc(fetch_dir, write_dir, workload):
+    """Average FETCH_SIZE / WRITE_SIZE (KiB per dispatch) of each measured
+    kernel, scaled by the factor that maps the calibration kernel's reading to
+    its known 1 GiB, into bytes per launch."""
+    out = {"workload": workload, "counters_kib_per_dispatch": {}, "calibration": {}, "kernels": {}}
+    scaled = {}
+    for counter, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        rows = _pmc_rows(d)
+        cal = [v for n, v in rows if CALIB_SUBSTR in n]
+        if not cal:
+            raise SystemExit(f"no calibration dispatch ({CALIB_SUBSTR}) in {d}")
+        factor = CALIB_BYTES / (max(cal) * 1024.0)
+        out["calibration"][counter] = {"kib_measured": max(cal), "bytes_known": CALIB_BYTES,
+                                       "factor": factor}
+        for tag, sub in PMC_KERNELS.items():
+            vals = [v for n, v in rows if sub in n]
+            if vals:
+                kib = sum(vals) / len(vals)
+                out["counters_kib_per_dispatch"].setdefault(tag, {})[counter] = kib
+                scaled.setdefault(tag, {})[counter] = kib * 1024.0 * factor
+    for tag, c in scaled.items():
+        if "FETCH_SIZE" in c and "WRITE_SIZE" in c:
+            out["kernels"][tag] = {"fetch_bytes": c["FETCH_SIZE"], "write_bytes": c["WRITE_SIZE"],
+                                   "traffic_bytes": c["FETCH_SIZE"] + c["WRITE_SIZE"]}
+    path = os.path.join(ROOT, "profiles", f"pmc_traffic_{workload}.json")
+    with open(path, "w") as fh:
+        json.dump(out, fh, indent=1)
+    print(json.dumps(out, indent=1))
+
+
+def measured_traffic(workload, tag):
+    """HBM bytes per launch of kernel `tag` from the committed PMC summary, or None."""
+    path = os.path.join(ROOT, "profiles", f"pmc_traffic_{workload}.json")
+    try:
+        with open(path) as fh:
+            d = json.load(fh)
+        return d["kernels"][tag]["traffic_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,7 +182,13 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-coils", type=int, default=2)
     ap.add_argument("--phases", default="aha,type2,type1")
+    ap.add_argument("--reduce-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="reduce two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of this script "
+                         "into the traffic summary profiles/pmc_traffic_<workload>.json and exit")
     args = ap.parse_args()
+    if args.reduce_pmc:
+        reduce_pmc(args.reduce_pmc[0], args.reduce_pmc[1], args.workload)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the coil-sharded multi-GPU design is documented, not built)", file=sys.stderr)
@@ -185,10 +260,11 @@ def main():
     if "type1" in phases:
         res["type1"] = run_phase("type1", lambda: p1.execute(y, back))
 
-    # a plain copy of known size, for scaling rocprof's FETCH_SIZE / WRITE_SIZE
-    calib = torch.empty(1 << 27, dtype=torch.complex64, device=dev)   # 1 GiB
+    # an elementwise kernel of known traffic (reads 1 GiB, writes 1 GiB), for
+    # scaling rocprof's FETCH_SIZE / WRITE_SIZE readings
+    calib = torch.zeros(CALIB_BYTES // 4, dtype=torch.float32, device=dev)
     calib_dst = torch.empty_like(calib)
-    calib_dst.copy_(calib)
+    torch.neg(calib, out=calib_dst)
     torch.cuda.synchronize()
     del calib, calib_dst
 
@@ -229,8 +305,10 @@ def main():
         dom = max(toep, key=lambda k: toep[k][0])
         t_dom = toep[dom][0] * 1e-3
         achieved = ab[dom] / t_dom / 1e9
+        traffic = measured_traffic(args.workload, dom)
         roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": round(traffic) if traffic else None,
                 "algorithmic_bytes_per_launch": ab[dom], "avg_launch_ms": round(toep[dom][0], 4)}
     kernels_json = {}
     for k, (ms, n) in kern_avg.items():

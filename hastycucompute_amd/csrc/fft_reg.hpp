@@ -171,6 +171,7 @@ template <class V> struct PassArgs {
     int ny;                             // line i -> (z = i / ny, y = i % ny)
     int dz_on_load;                     // deapodisation applied on load (1) or store (0)
     const V* psf; int64_t psf_o, psf_i, psf_s;
+    int bchunk;                         // PSF passes: batch elements per block (kernel read once)
     int accumulate, sum_batch;
     const V* tw;                        // tw[k] = exp(-2 pi i k / L)
     int ldp;                            // LDS stride per line (complex elements)
@@ -182,7 +183,7 @@ template <int L1, int L2> struct RegShape {
 };
 
 template <class V, int SIGN>
-__device__ inline V twiddle_mul(V v, const V* __restrict__ tw, int idx) {
+__device__ inline V twiddle_mul(V v, const V* tw, int idx) {
     V w = tw[idx];
     if (SIGN > 0) w.y = -w.y;
     return cmul(v, w);
@@ -222,9 +223,27 @@ __global__ void __launch_bounds__(512) fft2_pass_kernel(PassArgs<V> a) {
     const int i = it * TX + c;
     const bool line_ok = i < a.ni;
     V* xl = lds + c * a.ldp;   // this line's exchange area
+    // twiddle table W_L^k, k < L, staged once per block after the exchange areas
+    V* twl = lds + int64_t(TX) * a.ldp;
+    for (int k = tid; k < L; k += blockDim.x) twl[k] = a.tw[k];
+    __syncthreads();
 
     const bool sum_b = MODE == PM_ROW_STORE && a.sum_batch;
-    const int nb_loop = sum_b ? a.nbatch : 1;
+    // a PSF block runs bchunk consecutive batch elements of its lines, so
+    // each kernel value is read from HBM once per chunk, not once per element
+    const int64_t b_first = PSF ? bblk * a.bchunk : bblk;
+    const int nb_loop = sum_b ? a.nbatch
+                              : (PSF ? int(min(int64_t(a.bchunk), int64_t(a.nbatch) - b_first)) : 1);
+    // the kernel column of this thread's positions, the same for every
+    // batch element of the chunk: loaded once, issued together
+    V kv[PSF ? L2 : 1];
+    if constexpr (PSF) {
+        if (t < L1 && line_ok) {
+            const V* k = a.psf + int64_t(o) * a.psf_o + int64_t(i) * a.psf_i;
+#pragma unroll
+            for (int k2 = 0; k2 < L2; ++k2) kv[k2] = k[int64_t(t + L1 * k2) * a.psf_s];
+        }
+    }
     R zy = R(1);
     if (ROWS && line_ok && a.dx) {
         const int z = i / a.ny;
@@ -239,7 +258,15 @@ __global__ void __launch_bounds__(512) fft2_pass_kernel(PassArgs<V> a) {
         for (int m = 0; m < NACC; ++m) acc[m] = cmake<V>(R(0), R(0));
     }
     for (int bb = 0; bb < nb_loop; ++bb) {
-        const int64_t b = sum_b ? bb : bblk;
+        const int64_t b = sum_b ? bb : b_first + bb;
+        // per-iteration copies the compiler cannot hoist: otherwise the
+        // per-position addresses and masks of every load and store stay live
+        // in registers across the batch loop (and halve the occupancy)
+        int64_t in_s = a.in_s, out_s = a.out_s;
+        int in_shift = a.in_shift, in_len = a.in_len, out_shift = a.out_shift, out_len = a.out_len;
+        if (nb_loop > 1)
+            asm volatile("" : "+s"(in_s), "+s"(out_s), "+s"(in_shift), "+s"(in_len), "+s"(out_shift),
+                         "+s"(out_len));
         V v[T];
         // ---- load (step-1 layout: thread t = p2 holds p = L2 p1 + t) ----
         if (t < L2) {
@@ -247,11 +274,11 @@ __global__ void __launch_bounds__(512) fft2_pass_kernel(PassArgs<V> a) {
 #pragma unroll
             for (int p1 = 0; p1 < L1; ++p1) {
                 const int p = L2 * p1 + t;
-                int j = p - a.in_shift;
+                int j = p - in_shift;
                 j = j < 0 ? j + L : (j >= L ? j - L : j);
                 V val = cmake<V>(R(0), R(0));
-                if (line_ok && j < a.in_len) {
-                    val = src[int64_t(j) * a.in_s];
+                if (line_ok && j < in_len) {
+                    val = src[int64_t(j) * in_s];
                     if constexpr (MODE == PM_ROW_LOAD) {
                         if (a.lm1.ptr) val = diag_mult(val, a.lm1.ptr[b * a.lm1.sb + int64_t(i) * a.lm1.sr + j], a.lm1.type);
                         if (a.lm2.ptr) val = diag_mult(val, a.lm2.ptr[b * a.lm2.sb + int64_t(i) * a.lm2.sr + j], a.lm2.type);
@@ -262,7 +289,7 @@ __global__ void __launch_bounds__(512) fft2_pass_kernel(PassArgs<V> a) {
             }
             RegDFT<L1, SIGN, V>::run(v);
 #pragma unroll
-            for (int k1 = 1; k1 < L1; ++k1) v[k1] = twiddle_mul<V, SIGN>(v[k1], a.tw, t * k1);
+            for (int k1 = 1; k1 < L1; ++k1) v[k1] = twiddle_mul<V, SIGN>(v[k1], twl, t * k1);
 #pragma unroll
             for (int k1 = 0; k1 < L1; ++k1) xl[t * (L1 + 1) + k1] = v[k1];
         }
@@ -275,15 +302,14 @@ __global__ void __launch_bounds__(512) fft2_pass_kernel(PassArgs<V> a) {
         }
         if constexpr (PSF) {
             if (t < L1) {
-                const V* k = a.psf + int64_t(o) * a.psf_o + int64_t(i) * a.psf_i;
                 if (line_ok) {
 #pragma unroll
-                    for (int k2 = 0; k2 < L2; ++k2) v[k2] = cmul(v[k2], k[int64_t(t + L1 * k2) * a.psf_s]);
+                    for (int k2 = 0; k2 < L2; ++k2) v[k2] = cmul(v[k2], kv[k2]);
                 }
                 // inverse, factorisation L = L2 * L1: step 1 over the L2 held values
                 RegDFT<L2, -SIGN, V>::run(v);
 #pragma unroll
-                for (int k1 = 1; k1 < L2; ++k1) v[k1] = twiddle_mul<V, -SIGN>(v[k1], a.tw, t * k1);
+                for (int k1 = 1; k1 < L2; ++k1) v[k1] = twiddle_mul<V, -SIGN>(v[k1], twl, t * k1);
             }
             __syncthreads();   // step-2 reads of the first exchange are done
             if (t < L1) {
@@ -316,8 +342,8 @@ __global__ void __launch_bounds__(512) fft2_pass_kernel(PassArgs<V> a) {
 #pragma unroll
                     for (int m = 0; m < NACC; ++m) {
                         const int j = t + T * m;
-                        if (j < a.out_len) {
-                            int q = j + a.out_shift;
+                        if (j < out_len) {
+                            int q = j + out_shift;
                             q = q >= L ? q - L : (q < 0 ? q + L : q);
                             V val = cscale(xl[q], a.scale);
                             if (m1) val = diag_mult(val, m1[j], a.sm1.type);
@@ -328,8 +354,8 @@ __global__ void __launch_bounds__(512) fft2_pass_kernel(PassArgs<V> a) {
                         }
                     }
                 } else {
-                    for (int j = t; j < a.out_len; j += T) {
-                        int q = j + a.out_shift;
+                    for (int j = t; j < out_len; j += T) {
+                        int q = j + out_shift;
                         q = q >= L ? q - L : (q < 0 ? q + L : q);
                         V val = cscale(xl[q], a.scale);
                         if (m1) val = diag_mult(val, m1[j], a.sm1.type);
@@ -349,9 +375,9 @@ __global__ void __launch_bounds__(512) fft2_pass_kernel(PassArgs<V> a) {
 #pragma unroll
                 for (int k = 0; k < NOWN; ++k) {
                     const int q = t + OSTRIDE * k;
-                    int j = q - a.out_shift;
+                    int j = q - out_shift;
                     j = j < 0 ? j + L : (j >= L ? j - L : j);
-                    if (j < a.out_len) dst[int64_t(j) * a.out_s] = cscale(v[k], a.scale);
+                    if (j < out_len) dst[int64_t(j) * out_s] = cscale(v[k], a.scale);
                 }
             }
         }

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <atomic>
+#include <cstdlib>
 #include <mutex>
 
 #include "fft_reg.hpp"
@@ -14,10 +15,12 @@ namespace {
 template <class V, int L1, int L2, int SIGN, int MODE>
 void launch_one(const PassArgs<V>& a, const char* tag, hipStream_t s) {
     constexpr int T = RegShape<L1, L2>::T;
-    const int64_t blocks = ceil_div(a.ni, a.tx) * int64_t(a.no) *
-                           ((MODE == PM_ROW_STORE && a.sum_batch) ? 1 : int64_t(a.nbatch));
+    const int64_t nb_blocks = (MODE == PM_ROW_STORE && a.sum_batch) ? 1
+                              : MODE == PM_COL_PSF                ? ceil_div(a.nbatch, a.bchunk)
+                                                                  : int64_t(a.nbatch);
+    const int64_t blocks = ceil_div(a.ni, a.tx) * int64_t(a.no) * nb_blocks;
     if (blocks == 0) return;
-    const size_t lds = size_t(a.tx) * size_t(a.ldp) * sizeof(V);
+    const size_t lds = (size_t(a.tx) * size_t(a.ldp) + size_t(L1 * L2)) * sizeof(V);
     auto kern = fft2_pass_kernel<V, L1, L2, SIGN, MODE>;
     if (lds > 64 * 1024) {
         static std::once_flag once;
@@ -51,13 +54,30 @@ void launch_shape(PassArgs<V> a, int sign, int mode, const char* tag, hipStream_
     HASTY_REQUIRE(!(mode == PM_ROW_STORE && a.sum_batch) ||
                       a.out_len <= ((L / 2 + T - 1) / T) * T,
                   "coil-sum pass supports outputs up to half the line");
+    // HASTY_COL_TX / HASTY_PSF_TX: lines per block of the column passes
+    // (tuning; default 16 columns = 128 contiguous bytes of c64 per row)
+    static const int env_col_tx = std::getenv("HASTY_COL_TX") ? std::atoi(std::getenv("HASTY_COL_TX")) : 0;
+    static const int env_psf_tx = std::getenv("HASTY_PSF_TX") ? std::atoi(std::getenv("HASTY_PSF_TX")) : 0;
     if (rows) {
         a.tx = std::max(1, 256 / T);
+    } else if (mode == PM_COL_PSF && env_psf_tx > 0) {
+        a.tx = env_psf_tx;
+    } else if (env_col_tx > 0) {
+        a.tx = env_col_tx;
     } else if (a.tx <= 0) {
         a.tx = 16;
     }
-    while (a.tx > 1 && size_t(a.tx) * a.ldp * sizeof(V) > 160 * 1024) a.tx /= 2;
+    while (a.tx > 1 && (size_t(a.tx) * a.ldp + size_t(L)) * sizeof(V) > 160 * 1024) a.tx /= 2;
     while (a.tx * T > 512) a.tx /= 2;
+    if (mode == PM_COL_PSF) {
+        // batch elements per block: as many as keep >= 4096 blocks in the
+        // launch, so the kernel columns are read from HBM once per chunk
+        const int64_t line_blocks = ceil_div(a.ni, a.tx) * int64_t(a.no);
+        const int64_t nchunks = std::min<int64_t>(a.nbatch, std::max<int64_t>(1, ceil_div(4096, line_blocks)));
+        a.bchunk = int(ceil_div(a.nbatch, nchunks));
+    } else {
+        a.bchunk = 1;
+    }
     if (sign < 0) launch_mode<V, L1, L2, -1>(a, mode, tag, s);
     else launch_mode<V, L1, L2, 1>(a, mode, tag, s);
 }
