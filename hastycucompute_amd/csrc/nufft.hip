@@ -384,6 +384,8 @@ NufftPlan* nufft_makeplan(int type, int dim, const int64_t* nmodes, int sign, in
     p->ntiles = int64_t(p->ntile[0]) * p->ntile[1] * p->ntile[2];
     const int method = opts ? opts->method : 3;
     if (type == 2) {
+        // one transform per block by default: a second staged tile halves
+        // the occupancy, which costs more than re-reading the point data
         int group = (opts && opts->coil_group > 0) ? opts->coil_group : 1;
         group = std::min(std::min(group, kMaxGroupInterp), ntransf);
         while (group > 1 && interp_lds_bytes(p.get(), group) > kLdsMax) --group;
