@@ -53,6 +53,7 @@ SIGNATURES = {
     "hasty_nufft_setpts": (c_int, [c_vp, c_i64, c_vp, c_vp, c_vp]),
     "hasty_nufft_execute": (c_int, [c_vp, c_vp, c_vp]),
     "hasty_nufft_set_stream": (c_int, [c_vp, c_vp]),
+    "hasty_nufft_coord_grad": (c_int, [c_vp, c_vp, c_vp, c_vp]),
     "hasty_nufft_destroy": (c_int, [c_vp]),
     "hasty_nufft_info": (c_int, [c_vp, P_int, P_dbl, P_i64, P_i32, P_i64]),
     "hasty_toeplitz_kernel": (c_int, [c_int, P_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_vp]),

@@ -114,6 +114,13 @@ int hasty_nufft_destroy(hasty_nufft_plan plan) {
     return guarded([&] { delete reinterpret_cast<NufftPlan*>(plan); });
 }
 
+int hasty_nufft_coord_grad(hasty_nufft_plan plan, const void* h, const void* w, void* out) {
+    return guarded([&] {
+        HASTY_REQUIRE(plan != nullptr, "plan is NULL");
+        nufft_coord_grad(reinterpret_cast<NufftPlan*>(plan), h, w, out);
+    });
+}
+
 int hasty_nufft_info(hasty_nufft_plan plan, int* width, double* beta, int64_t* nf3, int32_t* tile3,
                      int64_t* device_bytes) {
     return guarded([&] {

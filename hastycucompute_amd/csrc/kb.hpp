@@ -51,4 +51,22 @@ template <class R> __device__ inline R kb_phi(R t, R inv_half_w, R beta) {
     return z > R(0) ? kb_i0(beta * sqrt(z)) : (z == R(0) ? R(1) : R(0));
 }
 
+// d phi / dt inside the support.  With c = 2/w and s = sqrt(1 - (c t)^2),
+// d/dt I0(beta s) = I1(beta s) beta ds/dt = -beta^2 c^2 t [I1(z) / z]_{z = beta s},
+// and I1(z)/z = 1/2 sum_k (z^2/4)^k / (k! (k+1)!)  (A&S 9.6.10), finite at s = 0.
+template <class R> __device__ inline R kb_dphi(R t, R inv_half_w, R beta) {
+    const R ct = t * inv_half_w;
+    const R z2 = R(1) - ct * ct;
+    if (!(z2 >= R(0))) return R(0);
+    const R q = R(0.25) * beta * beta * z2;
+    R term = R(0.5), sum = R(0.5);
+    const R stop = sizeof(R) == 4 ? R(6e-8) : R(1e-16);
+    for (int k = 1; k < 80; ++k) {
+        term *= q / R(k * (k + 1));
+        sum += term;
+        if (term < stop * sum) break;
+    }
+    return -beta * beta * inv_half_w * inv_half_w * t * sum;
+}
+
 }  // namespace hasty

@@ -700,6 +700,66 @@ void execute_impl(NufftPlan* p, V* c, V* f) {
 
 }  // namespace
 
+namespace {
+
+template <class V>
+void coord_grad_impl(NufftPlan* p, const V* h, const V* w, decltype(V::x)* out) {
+    using R = decltype(V::x);
+    const hipStream_t s = p->stream;
+    const int64_t nimg = p->n[0] * p->n[1] * p->n[2];
+    PadGeometry pg{};
+    for (int a = 0; a < 3; ++a) {
+        pg.n[a] = p->n[a];
+        pg.L[a] = p->nf[a];
+        pg.shift_in[a] = int(-(p->n[a] / 2));
+        pg.shift_out[a] = int(-(p->n[a] / 2));
+    }
+    const R* deapod[3] = {static_cast<const R*>(p->deapod[0]), static_cast<const R*>(p->deapod[1]),
+                          static_cast<const R*>(p->deapod[2])};
+    V* fine = static_cast<V*>(p->fine);
+    for (int b0 = 0; b0 < p->ntransf; b0 += p->max_batch) {
+        const int nb = std::min(p->max_batch, p->ntransf - b0);
+        DiagOp<V> none{};
+        pruned_forward<V>(pg, nb, h + int64_t(b0) * nimg, fine, static_cast<V*>(p->bufA),
+                          static_cast<V*>(p->bufB), p->sign, none, none, deapod, s);
+        if (p->M == 0) continue;
+        CoordGradArgs<V> a{};
+        a.fine = fine;
+        a.w = w + int64_t(b0) * p->M;
+        a.out = out;
+        a.sidx = p->sidx;
+        a.sl0 = p->sl0;
+        a.sfrac = static_cast<const R*>(p->sfrac);
+        a.sw = static_cast<const R*>(p->sw);
+        a.M = p->M;
+        a.nbatch = nb;
+        a.ws = weight_stride(p->w);
+        a.dim = p->dim;
+        a.accumulate = b0 > 0;
+        a.beta = p->beta;
+        a.g = geo_of(p);
+        HASTY_PROF("nufft_coord_grad", s);
+        hipLaunchKernelGGL(reinterpret_cast<void (*)(CoordGradArgs<V>)>(kernel_ptr(p, 4)),
+                           dim3(unsigned(ceil_div(p->M, 256))), dim3(256), 0, s, a);
+        HASTY_CHECK_HIP(hipGetLastError());
+    }
+}
+
+}  // namespace
+
+void nufft_coord_grad(NufftPlan* p, const void* h, const void* w, void* out) {
+    if (!p->have_pts) throw Error(HASTY_ERR_STATE, "nufft coordinate gradient called before setpts");
+    HASTY_REQUIRE(p->type == 2, "the coordinate gradient runs on a type-2 plan");
+    HASTY_REQUIRE(h != nullptr && out != nullptr, "mode or output array is NULL");
+    HASTY_REQUIRE(w != nullptr || p->M == 0, "nonuniform array is NULL");
+    if (p->f64)
+        coord_grad_impl<double2>(p, static_cast<const double2*>(h), static_cast<const double2*>(w),
+                                 static_cast<double*>(out));
+    else
+        coord_grad_impl<float2>(p, static_cast<const float2*>(h), static_cast<const float2*>(w),
+                                static_cast<float*>(out));
+}
+
 void nufft_execute(NufftPlan* p, void* c, void* f) {
     if (!p->have_pts) throw Error(HASTY_ERR_STATE, "nufft execute called before setpts");
     HASTY_REQUIRE(f != nullptr, "mode array is NULL");

@@ -82,6 +82,8 @@ NufftPlan* nufft_makeplan(int type, int dim, const int64_t* nmodes, int sign, in
                           double tol, bool f64, const hasty_nufft_opts* opts, hipStream_t stream);
 void nufft_setpts(NufftPlan* p, int64_t M, const void* c0, const void* c1, const void* c2);
 void nufft_execute(NufftPlan* p, void* c, void* f);
+// out[a][j] = Re sum_t conj(w[t][j]) d/dk_a (type2(h)_t)_j on a type-2 plan
+void nufft_coord_grad(NufftPlan* p, const void* h, const void* w, void* out);
 
 // Pruned multi-dimensional FFT helpers shared with the Toeplitz code.
 // Geometry in slots (z, y, x): image extent n, FFT length L, shifts.

@@ -9,7 +9,8 @@ template <> void* nufft_kernel_ptr<float2>(int which, int w) {
     case Wv:                                                                                \
         return which == 0   ? reinterpret_cast<void*>(&interp_kernel<float2, Wv>)           \
                : which == 2 ? reinterpret_cast<void*>(&interp_gm_kernel<float2, Wv>)        \
-                            : reinterpret_cast<void*>(&spread_gm_kernel<float2, Wv>);
+               : which == 3 ? reinterpret_cast<void*>(&spread_gm_kernel<float2, Wv>)        \
+                            : reinterpret_cast<void*>(&coord_grad_kernel<float2, Wv>);
         HASTY_CASE(2) HASTY_CASE(3) HASTY_CASE(4) HASTY_CASE(5) HASTY_CASE(6) HASTY_CASE(7)
         HASTY_CASE(8) HASTY_CASE(9)
 #undef HASTY_CASE

@@ -9,7 +9,8 @@ template <> void* nufft_kernel_ptr<double2>(int which, int w) {
     case Wv:                                                                                \
         return which == 0   ? reinterpret_cast<void*>(&interp_kernel<double2, Wv>)           \
                : which == 2 ? reinterpret_cast<void*>(&interp_gm_kernel<double2, Wv>)        \
-                            : reinterpret_cast<void*>(&spread_gm_kernel<double2, Wv>);
+               : which == 3 ? reinterpret_cast<void*>(&spread_gm_kernel<double2, Wv>)        \
+                            : reinterpret_cast<void*>(&coord_grad_kernel<double2, Wv>);
         HASTY_CASE(2) HASTY_CASE(3) HASTY_CASE(4) HASTY_CASE(5) HASTY_CASE(6) HASTY_CASE(7)
         HASTY_CASE(8) HASTY_CASE(9) HASTY_CASE(10) HASTY_CASE(11) HASTY_CASE(12) HASTY_CASE(13)
         HASTY_CASE(14)

@@ -418,6 +418,102 @@ __global__ void zero_tiles_kernel(V* fine, const int* tiles, int nbatch, Geo g) 
 }
 
 // which: 0 interp (LDS), 1 spread (LDS), 2 interp (GM), 3 spread (GM)
+// ---------------------------------------------------------------------------
+// Trajectory gradient by kernel-derivative interpolation.  With the fine grid
+// F_t = FFT(deapodised, padded h_t) of a type-2 plan and u_a = k_a nf_a / 2 pi,
+//   d/dk_a (type2(h)_t)_j = nf_a / (2 pi) sum_l d_a phi(u_j - l) F_t[l],
+// where d_a phi differentiates the window along slot a only.  Each thread
+// takes one sorted point, forms the three derivatives separably (x-row sums
+// of phi and phi', then y, then z) for every transform t and accumulates
+//   out[a][j] (+)= Re sum_t conj(w_t[j]) d/dk_a (type2(h)_t)_j.
+// ---------------------------------------------------------------------------
+template <class V>
+struct CoordGradArgs {
+    using R = decltype(V::x);
+    const V* fine;          // [nbatch, nf]
+    const V* w;             // [nbatch, M] original point order
+    R* out;                 // [dim, M] original point order
+    const int* sidx;
+    const int* sl0;         // [3][M]
+    const R* sfrac;         // [3][M]
+    const R* sw;            // [M][ws]
+    int64_t M;
+    int nbatch, ws, dim, accumulate;
+    double beta;
+    Geo g;
+};
+
+template <class V, int W>
+__global__ void __launch_bounds__(256) coord_grad_kernel(CoordGradArgs<V> a) {
+    using R = decltype(V::x);
+    const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s >= a.M) return;
+    const R ihw = R(2.0 / double(W));
+    const R beta = R(a.beta);
+    R wv[3][W], dw[3][W];
+    int idx[3][W];
+    for (int d = 0; d < 3; ++d) {
+        const int l0 = a.sl0[int64_t(d) * a.M + s];
+        const R fr = a.sfrac[int64_t(d) * a.M + s];
+        const int n = int(a.g.nf[d]);
+#pragma unroll
+        for (int q = 0; q < W; ++q) {
+            int l = l0 + q;
+            l -= l >= n ? n : 0;
+            idx[d][q] = l;
+            wv[d][q] = a.sw[s * a.ws + d * W + q];
+            dw[d][q] = a.g.wa[d] == 1 ? R(0) : kb_dphi<R>(fr - R(q), ihw, beta);
+        }
+    }
+    const int orig = a.sidx[s];
+    const int64_t G = a.g.nf[0] * a.g.nf[1] * a.g.nf[2];
+    R acc[3] = {R(0), R(0), R(0)};
+    for (int t = 0; t < a.nbatch; ++t) {
+        const V* g = a.fine + int64_t(t) * G;
+        V Dz = cmake<V>(R(0), R(0)), Dy = Dz, Dx = Dz;
+        for (int qz = 0; qz < a.g.wa[0]; ++qz) {
+            V S = cmake<V>(R(0), R(0)), Sdy = S, Sdx = S;
+#pragma unroll
+            for (int qy = 0; qy < W; ++qy) {
+                if (qy < a.g.wa[1]) {
+                    const V* row = g + (int64_t(idx[0][qz]) * a.g.nf[1] + idx[1][qy]) * a.g.nf[2];
+                    V sx = cmake<V>(R(0), R(0)), sdx = sx;
+#pragma unroll
+                    for (int qx = 0; qx < W; ++qx) {
+                        const V f = row[idx[2][qx]];
+                        sx.x += wv[2][qx] * f.x;
+                        sx.y += wv[2][qx] * f.y;
+                        sdx.x += dw[2][qx] * f.x;
+                        sdx.y += dw[2][qx] * f.y;
+                    }
+                    S.x += wv[1][qy] * sx.x;
+                    S.y += wv[1][qy] * sx.y;
+                    Sdy.x += dw[1][qy] * sx.x;
+                    Sdy.y += dw[1][qy] * sx.y;
+                    Sdx.x += wv[1][qy] * sdx.x;
+                    Sdx.y += wv[1][qy] * sdx.y;
+                }
+            }
+            Dz.x += dw[0][qz] * S.x;
+            Dz.y += dw[0][qz] * S.y;
+            Dy.x += wv[0][qz] * Sdy.x;
+            Dy.y += wv[0][qz] * Sdy.y;
+            Dx.x += wv[0][qz] * Sdx.x;
+            Dx.y += wv[0][qz] * Sdx.y;
+        }
+        const V c = a.w[int64_t(t) * a.M + orig];
+        // Re(conj(c) D) = c.x D.x + c.y D.y
+        acc[0] += c.x * Dz.x + c.y * Dz.y;
+        acc[1] += c.x * Dy.x + c.y * Dy.y;
+        acc[2] += c.x * Dx.x + c.y * Dx.y;
+    }
+    for (int d = 3 - a.dim; d < 3; ++d) {
+        const R v = acc[d] * R(double(a.g.nf[d]) * 0.15915494309189533576888376337251);
+        R* o = a.out + int64_t(d - (3 - a.dim)) * a.M + orig;
+        *o = a.accumulate ? *o + v : v;
+    }
+}
+
 template <class V> void* nufft_kernel_ptr(int which, int w);
 // spreading kernel for CG transforms per block (1, 2, 4, 8, 16, 32 / 16 for complex128)
 template <class V> void* nufft_spread_ptr(int cg);

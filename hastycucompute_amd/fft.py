@@ -209,6 +209,31 @@ class nufft_plan:
             check(lib().hasty_nufft_set_stream(self._handle(), _stream(self._device)))
             check(lib().hasty_nufft_execute(self._handle(), c.data_ptr(), f.data_ptr()))
 
+    def coord_grad(self, h: torch.Tensor, w: torch.Tensor) -> list:
+        """Type-2 plans: ``out[a][j] = Re sum_t conj(w[t, j]) d/dk_{j,a} type2(h)[t, j]``
+        for the coordinates of the last ``setpts`` (one real tensor [M] per
+        coordinate, in the caller's point order), by interpolation with the
+        derivative of the window.  No counterpart in the reference."""
+        if self._type != nufft_type.TYPE_2:
+            raise RuntimeError("the coordinate gradient runs on a type-2 plan")
+        if self._M is None:
+            raise RuntimeError("setpts not called")
+        nt = int(self._opts.ntransf)
+        modes = tuple(int(m) for m in self._opts.nmodes)
+        for name, t in (("h", h), ("w", w)):
+            _require_gpu(t, name)
+            if t.dtype != self._cdtype:
+                raise RuntimeError(f"{name} must have dtype {self._cdtype}")
+        if tuple(h.shape) != (nt,) + modes:
+            raise RuntimeError(f"h must have shape {(nt,) + modes}")
+        if tuple(w.shape) != (nt, self._M):
+            raise RuntimeError(f"w must have shape [{nt}, {self._M}]")
+        out = torch.empty((self._dim, self._M), dtype=self._dtype, device=self._device)
+        with torch.cuda.device(self._device):
+            check(lib().hasty_nufft_set_stream(self._handle(), _stream(self._device)))
+            check(lib().hasty_nufft_coord_grad(self._handle(), h.data_ptr(), w.data_ptr(), out.data_ptr()))
+        return [out[a] for a in range(self._dim)]
+
     def info(self) -> dict:
         w = ctypes.c_int()
         beta = ctypes.c_double()

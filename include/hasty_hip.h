@@ -91,6 +91,16 @@ int hasty_nufft_setpts(hasty_nufft_plan plan, int64_t M, const void* c0, const v
  * and writes c. */
 int hasty_nufft_execute(hasty_nufft_plan plan, void* c, void* f);
 int hasty_nufft_set_stream(hasty_nufft_plan plan, void* stream);
+/* Trajectory gradient (no counterpart in the reference, whose NUFFT is not
+ * differentiable; this backs the d/dtrajectory autograd of the Python layer).
+ * On a type-2 plan with points set:
+ *   out[a][j] = Re sum_t conj(w[t][j]) d/dk_{j,a} (type2(h)_t)_j,
+ * h = modes [ntransf, n...], w = nonuniform [ntransf, M], out = real [dim, M]
+ * (float for HASTY_F32 plans, double for HASTY_F64) in the caller's point
+ * order, by interpolation with the derivative of the window on the fine grid
+ * of h.  For y = type2(f): grad_k = out(h = f, w = dL/dy); for f = type1(c)
+ * with the opposite sign: grad_k = out(h = dL/df, w = c). */
+int hasty_nufft_coord_grad(hasty_nufft_plan plan, const void* h, const void* w, void* out);
 int hasty_nufft_destroy(hasty_nufft_plan plan);
 /* introspection: width, beta, fine grid, tiles, device bytes held */
 int hasty_nufft_info(hasty_nufft_plan plan, int* width, double* beta, int64_t* nf3,
