@@ -11,6 +11,10 @@ iterations bracketed by torch.cuda.synchronize()):
   * aha   -- one SENSE Toeplitz normal-operator apply over all coils (value)
   * type2 -- one forward NUFFT of all coils (Mpts/s = C*M / t)
   * type1 -- one adjoint NUFFT of all coils
+  * plain -- the same three on the plain per-coil path (one coil per launch
+             sequence / one single-transform plan looped over the coils)
+  * frames -- batched dynamic frames: 20 frames of 96^3 x 16 coils with
+             uneven trajectories, 15 CG iterations each, eager vs hipGraph
 Per-kernel launch times come from HIP events the library records on its own
 stream (hasty_profile_*), used for the roofline of the dominant kernel.
 Accuracy is checked in the same run in FP64 on the GPU (direct NDFT at a
@@ -173,6 +177,68 @@ def measured_traffic(workload, tag):
         return None
 
 
+def bench_frames(dev, iters=15, nframes=20, N=96, C=16, samples=192, seed=3, reps=3):
+    """Batched dynamic frames: ``nframes`` rotated golden-means kooshballs of
+    U[500, 3000] spokes x ``samples`` at N^3 with C coils, 15 CG iterations
+    per frame on each frame's Toeplitz normal operator; eager launches vs one
+    captured hipGraph iteration replayed.  Returns the timings."""
+    import numpy as np
+    import torch
+
+    from hastycucompute_amd import autotune, synth
+    from hastycucompute_amd import fft as hf
+    from hastycucompute_amd.frames import FrameBatch
+
+    rng = np.random.default_rng(seed)
+    spokes = rng.integers(500, 3001, nframes)
+    n = (N, N, N)
+    S = torch.from_numpy(synth.coil_maps(n, C, seed=1)).to(dev)
+    x = torch.from_numpy(synth.image(n, seed=0)).to(dev)
+    coords = [[torch.from_numpy(c).to(dev) for c in synth.rotated_kooshball(int(s), samples, f)]
+              for f, s in enumerate(spokes)]
+    t0 = time.perf_counter()
+    tuned = autotune.tune_nufft(n, C, coords[0], hf.nufft_type.TYPE_1, tol=1e-5, reps=2)
+    tune_s = time.perf_counter() - t0
+    fwd = hf.nufft_plan.make(hf.nufft_opts(nmodes=n, ntransf=C, tol=1e-5, sign=-1), hf.nufft_type.TYPE_2)
+    kdata = []
+    img = (S * x[None]).contiguous()
+    for f in range(nframes):
+        fwd.setpts(coords[f])
+        y = torch.empty((C, int(coords[f][0].shape[0])), dtype=torch.complex64, device=dev)
+        fwd.execute(img, y)
+        kdata.append(y + 0.01 * y.abs().mean() * torch.randn_like(y))
+    fwd.free()
+    t0 = time.perf_counter()
+    fb = FrameBatch(S, coords, kdata, tol=1e-5, nufft_opts=autotune.plan_options(tuned))
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    out = {}
+    for mode in ("eager", "graph"):
+        fb.solve(iters, graph=(mode == "graph"))       # warm (and capture)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fb.solve(iters, graph=(mode == "graph"))
+        torch.cuda.synchronize()
+        out[mode] = (time.perf_counter() - t) / reps
+    xe = fb.solve(iters, graph=False).clone()
+    xg = fb.solve(iters, graph=True).clone()
+    same = bool(torch.equal(xe, xg))
+    total_pts = int(sum(int(c[0].shape[0]) for c in coords))
+    log(f"[bench] frames: {nframes} x {N}^3 x {C} coils, {total_pts} pts, {iters} CG iters: "
+        f"eager {out['eager'] * 1e3:.1f} ms, graph {out['graph'] * 1e3:.1f} ms, identical={same}")
+    return {"frames": nframes, "image": [N] * 3, "coils": C, "cg_iters": iters,
+            "spokes_per_frame": [int(s) for s in spokes], "samples_per_spoke": samples,
+            "total_points_per_coil": total_pts,
+            "eager_ms_per_solve": round(out["eager"] * 1e3, 3),
+            "graph_ms_per_solve": round(out["graph"] * 1e3, 3),
+            "graph_speedup": round(out["eager"] / out["graph"], 3),
+            "frame_iterations_per_s": round(nframes * iters / out["graph"], 1),
+            "graph_equals_eager": same,
+            "setup_s": round(setup_s, 2), "autotune_s": round(tune_s, 2),
+            "autotuned_type1_opts": autotune.plan_options(tuned)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -181,7 +247,7 @@ def main():
     ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-coils", type=int, default=2)
-    ap.add_argument("--phases", default="aha,type2,type1")
+    ap.add_argument("--phases", default="aha,type2,type1,plain,frames")
     ap.add_argument("--reduce-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="reduce two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of this script "
                          "into the traffic summary profiles/pmc_traffic_<workload>.json and exit")
@@ -260,6 +326,31 @@ def main():
     if "type1" in phases:
         res["type1"] = run_phase("type1", lambda: p1.execute(y, back))
 
+    # the plain per-coil GPU path, as the reference runs it: one coil per
+    # normal-operator launch sequence, one single-transform NUFFT plan looped
+    # over the coils (nufft.cppm:316-382 style, without its host copies)
+    plain = {}
+    if "plain" in phases:
+        op1 = NORMAL_T_T1_OP(kernel, smaps, coil_batch=1)
+        plain["aha"] = run_phase("plain per-coil aha", lambda: op1.apply(x, out))
+        del op1
+        one = dict(nmodes=(N, N, N), ntransf=1, tol=1e-5)
+        q2 = hf.nufft_plan.make(hf.nufft_opts(sign=-1, **one), hf.nufft_type.TYPE_2)
+        q1 = hf.nufft_plan.make(hf.nufft_opts(sign=+1, **one), hf.nufft_type.TYPE_1)
+        q2.setpts(coords)
+        q1.setpts(coords)
+        y_views = [y[c:c + 1] for c in range(C)]
+        img_views = [img_c[c:c + 1] for c in range(C)]
+        back_views = [back[c:c + 1] for c in range(C)]
+
+        def per_coil(plan, ins, outs):
+            for a, b in zip(ins, outs):
+                plan.execute(a, b)
+        plain["type2"] = run_phase("plain per-coil type2", lambda: per_coil(q2, img_views, y_views))
+        plain["type1"] = run_phase("plain per-coil type1", lambda: per_coil(q1, y_views, back_views))
+        q1.free()
+        q2.free()
+
     # an elementwise kernel of known traffic (reads 1 GiB, writes 1 GiB), for
     # scaling rocprof's FETCH_SIZE / WRITE_SIZE readings
     calib = torch.zeros(CALIB_BYTES // 4, dtype=torch.float32, device=dev)
@@ -318,6 +409,10 @@ def main():
             e["frac_of_8TBs"] = round(ab[k] / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
         kernels_json[k] = e
 
+    frames = None
+    if "frames" in phases:
+        frames = bench_frames(dev)
+
     cpu = None
     if not args.no_cpu_baseline and "aha" in phases:
         log("[bench] CPU baseline ...")
@@ -342,6 +437,13 @@ def main():
                    "kb_width": p2.info()["width"], "fine_grid": list(p2.info()["nf"])[-3:]},
         "nufft_type2_mpts_s": round(C * M / res["type2"] / 1e6, 1) if "type2" in res else None,
         "nufft_type1_mpts_s": round(C * M / res["type1"] / 1e6, 1) if "type1" in res else None,
+        "plain_per_coil_gpu": ({
+            "aha_applies_s": round(1.0 / plain["aha"], 3),
+            "nufft_type2_mpts_s": round(C * M / plain["type2"] / 1e6, 1),
+            "nufft_type1_mpts_s": round(C * M / plain["type1"] / 1e6, 1),
+            "batched_speedup": {k: round(plain[k] / res[k], 3) for k in plain if k in res},
+        } if plain else None),
+        "batched_frames_hipgraph": frames,
         "aha_pipeline_roofline": ({"achieved": round(ab["aha"] / aha_t / 1e9, 1), "peak": HBM_PEAK_GBS,
                                    "unit": "GB/s", "frac": round(ab["aha"] / aha_t / 1e9 / HBM_PEAK_GBS, 4),
                                    "algorithmic_bytes": ab["aha"]} if aha_t else None),
