@@ -16,11 +16,11 @@ loss with respect to a complex tensor z is the conjugate Wirtinger
 derivative scaled by 2, so for a linear map y = A x the input gradient is
 A^H g, and for a real parameter k it is Re(conj(g) dy/dk).
 
-``sense_cg_solve`` differentiates through a SENSE-CG reconstruction by the
-implicit-function theorem: one adjoint CG solve on the same Toeplitz normal
-operator instead of backpropagating through the iterations.  Only the
-converged iterate is checkpointed for the backward pass (the operator and
-its PSF are kept), so memory does not grow with the iteration count.
+``sense_cg_solve`` differentiates through a SENSE-CG reconstruction, either
+exactly for the truncated solve (reverse mode through the checkpointed CG
+iterates: one normal-operator apply and two kernel-derivative
+interpolations per iteration) or, for a converged solve, by the implicit
+function theorem (one adjoint CG solve on the same Toeplitz operator).
 """
 from __future__ import annotations
 
@@ -30,6 +30,7 @@ import torch
 
 from . import fft as hf
 from . import op as hop
+from ._lib import CG_STATE_DOUBLES, check, lib
 from .mri import NORMAL_T_T1_OP
 
 
@@ -187,8 +188,137 @@ class _SenseCG(torch.autograd.Function):
         return (gy, None, None, None, None, *gk)
 
 
+class _SenseCGAdjoint(torch.autograd.Function):
+    """x_K = K steps of CG on N(k) x = b(k) from x_0 = 0, differentiated in
+    reverse mode through the iterations (the exact gradient of the truncated
+    solve).  The forward pass checkpoints every iterate it needs -- p_i,
+    q_i = N p_i, r_{i+1} and the FP64 scalars rz_i, pq_i, alpha_i, rz_{i+1},
+    beta_i -- and the backward pass runs the adjoint recurrence of
+      q = N p, pq = Re<p,q>, alpha = rz/pq, x' = x + alpha p, r' = r - alpha q,
+      rz' = ||r'||^2, beta = rz'/rz, p' = r' + beta p
+    from the last step to the first: one normal-operator apply per step for
+    the vector adjoint, and for the trajectory
+      d/dk Re<g_q, N(k) p> = (1/N^d)[Re<A S p, dA(S g_q)> + Re<A S g_q, dA(S p)>]
+    through the kernel-derivative interpolation; finally b(k) = (1/N^d)
+    sum_c conj(S_c) A^H y_c contributes (1/N^d) Re<y, dA(S g_b)>."""
+
+    @staticmethod
+    def forward(ctx, y, smaps, op, iters, kernel_tol, *coords):
+        op.set_coords(coords)
+        nvox = float(torch.tensor(op.nmodes).prod())
+        dev = y.device
+        kernel = torch.empty(tuple(2 * n for n in op.nmodes), dtype=torch.complex64, device=dev)
+        hf.toeplitz_kernel(list(coords), kernel, tol=kernel_tol)
+        N = NORMAL_T_T1_OP(kernel, smaps.contiguous())
+        b = ((smaps.conj() * op.adjoint(y)).sum(0) / nvox).contiguous()
+        n = b.numel()
+        s = hf._stream(dev)
+        x = torch.zeros_like(b)
+        r = b.clone()
+        p = torch.empty_like(b)
+        q = torch.empty_like(b)
+        state = torch.zeros(CG_STATE_DOUBLES, dtype=torch.float64, device=dev)
+        check(lib().hasty_cg_init(r.data_ptr(), p.data_ptr(), n, 0.0, state.data_ptr(), s))
+        K = int(iters)
+        P, Q, R = [], [], []
+        scal = torch.zeros((K, 5), dtype=torch.float64, device=dev)   # rz_i, pq_i, alpha_i, rz_i+1, beta_i
+        for i in range(K):
+            P.append(p.clone())
+            scal[i, 0] = state[0]
+            N.apply(p, q)
+            Q.append(q.clone())
+            check(lib().hasty_cg_step(x.data_ptr(), r.data_ptr(), p.data_ptr(), q.data_ptr(), n,
+                                      state.data_ptr(), s))
+            scal[i, 1:5] = state[1:5]
+            R.append(r.clone())
+        ctx.op, ctx.N, ctx.nvox = op, N, nvox
+        ctx.P, ctx.Q, ctx.R, ctx.scal = P, Q, R, scal
+        ctx.save_for_backward(y, smaps, b, *coords)
+        return x
+
+    @staticmethod
+    def backward(ctx, gx):
+        y, smaps, b, *coords = ctx.saved_tensors
+        op, N, V = ctx.op, ctx.N, ctx.nvox
+        op.set_coords(coords)
+        want_k = any(ctx.needs_input_grad[5:])
+        P, Q, R, scal = ctx.P, ctx.Q, ctx.R, ctx.scal
+        K = len(P)
+        f32 = lambda t: t.to(torch.float32)     # noqa: E731  (FP64 device scalar -> FP32 factor)
+        gx = gx.contiguous()
+        gr = torch.zeros_like(gx)
+        gp = torch.zeros_like(gx)
+        grz = torch.zeros((), dtype=torch.float64, device=gx.device)
+        gk = [torch.zeros_like(c, dtype=op.dtype) for c in coords] if want_k else None
+        Np = torch.empty_like(gx)
+        for i in range(K - 1, -1, -1):
+            rz, pq, alpha, rz1, beta = scal[i, 0], scal[i, 1], scal[i, 2], scal[i, 3], scal[i, 4]
+            p_i, q_i, r1 = P[i], Q[i], R[i]
+            # p' = r' + beta p
+            gr = gr + gp
+            gbeta = _rdot64(gp, p_i)
+            gp_i = gp * f32(beta)
+            # beta = rz'/rz
+            grz1 = grz + gbeta / rz
+            grz_i = -gbeta * beta / rz
+            # rz' = ||r'||^2
+            gr = gr + r1 * f32(2 * grz1)
+            # r' = r - alpha q
+            galpha = -_rdot64(gr, q_i)
+            gq = gr * f32(-alpha)
+            # x' = x + alpha p
+            galpha = galpha + _rdot64(gx, p_i)
+            gp_i = gp_i + gx * f32(alpha)
+            # alpha = rz / pq
+            grz_i = grz_i + galpha / pq
+            gpq = -galpha * alpha / pq
+            # pq = Re<p, q>
+            gp_i = gp_i + q_i * f32(gpq)
+            gq = gq + p_i * f32(gpq)
+            # q = N p
+            N.apply(gq.contiguous(), Np)
+            gp_i = gp_i + Np
+            if want_k:
+                Sg = (smaps * gq.unsqueeze(0)).contiguous()
+                Sp = (smaps * p_i.unsqueeze(0)).contiguous()
+                t1 = op.coord_grad(Sg, op.forward(Sp))
+                t2 = op.coord_grad(Sp, op.forward(Sg))
+                for a in range(len(gk)):
+                    gk[a] += (t1[a] + t2[a]) / V
+            gp, grz = gp_i, grz_i
+            # gr and gx carry over unchanged (r = r' + alpha q, x = x' - alpha p)
+        # r_0 = b, p_0 = r_0, rz_0 = ||r_0||^2
+        gb = gr + gp + b * f32(2 * grz)
+        Sb = (smaps * gb.unsqueeze(0)).contiguous()
+        gy = op.forward(Sb) / V if ctx.needs_input_grad[0] else None
+        if want_k:
+            t = op.coord_grad(Sb, y.contiguous())
+            for a in range(len(gk)):
+                gk[a] += t[a] / V
+        else:
+            gk = [None] * len(coords)
+        return (gy, None, None, None, None, *gk)
+
+
+def _rdot64(a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
+    """Re <a, b> accumulated in FP64 on the device (0-d float64 tensor)."""
+    out = torch.empty(2, dtype=torch.float64, device=a.device)
+    aa, bb = a.contiguous(), b.contiguous()
+    check(lib().hasty_dot(aa.data_ptr(), bb.data_ptr(), aa.numel(), out.data_ptr(), hf._stream(a.device)))
+    return out[0]
+
+
 def sense_cg_solve(y: torch.Tensor, smaps: torch.Tensor, op: NufftOperator, *coords: torch.Tensor,
-                   iters: int = 25, kernel_tol: float = 1e-9) -> torch.Tensor:
-    """x_hat = argmin ||A(k) x - y||^2 by ``iters`` CG steps on the Toeplitz
-    normal operator; differentiable in y and k (implicit adjoint)."""
-    return _SenseCG.apply(y, smaps, op, iters, kernel_tol, *coords)
+                   iters: int = 25, kernel_tol: float = 1e-9, method: str = "adjoint") -> torch.Tensor:
+    """x_hat = ``iters`` CG steps on the Toeplitz normal equations of
+    ||A(k) x - y||^2 from x = 0, differentiable in y and k.
+
+    ``method="adjoint"`` (default) differentiates the truncated solve exactly,
+    in reverse mode through the checkpointed iterates; ``method="implicit"``
+    returns the gradient of the converged solution (one adjoint CG solve,
+    valid once the forward solve has converged)."""
+    if method == "adjoint":
+        return _SenseCGAdjoint.apply(y, smaps, op, iters, kernel_tol, *coords)
+    if method == "implicit":
+        return _SenseCG.apply(y, smaps, op, iters, kernel_tol, *coords)
+    raise RuntimeError("method must be 'adjoint' or 'implicit'")

@@ -87,3 +87,35 @@ def spiral2d(ninterleaves: int, nsamples: int, density_power: float = 1.5):
     y = np.clip(np.concatenate(out_y), -np.pi, np.pi - 1e-6)
     x = np.clip(np.concatenate(out_x), -np.pi, np.pi - 1e-6)
     return [y.astype(np.float32), x.astype(np.float32)]
+
+
+def ellipse_phantom(shape, nellipses: int = 10, seed: int = 4, texture: float = 0.1) -> np.ndarray:
+    """2-D complex test image: a sum of random ellipses plus a smooth
+    Gaussian texture (complex64)."""
+    rng = np.random.default_rng(seed)
+    ny, nx = shape
+    yy, xx = np.meshgrid(np.linspace(-1, 1, ny), np.linspace(-1, 1, nx), indexing="ij")
+    img = np.zeros(shape, np.complex128)
+    for _ in range(nellipses):
+        cy, cx = rng.uniform(-0.5, 0.5, 2)
+        ay, ax = rng.uniform(0.1, 0.45, 2)
+        th = rng.uniform(0, np.pi)
+        c, s = np.cos(th), np.sin(th)
+        u = (c * (xx - cx) + s * (yy - cy)) / ax
+        v = (-s * (xx - cx) + c * (yy - cy)) / ay
+        img += rng.uniform(0.2, 1.0) * np.exp(1j * rng.uniform(0, 2 * np.pi)) * (u * u + v * v <= 1.0)
+    noise = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    kern = np.exp(-0.5 * ((yy / 0.05) ** 2 + (xx / 0.05) ** 2))
+    smooth = np.fft.ifft2(np.fft.fft2(noise) * np.fft.fft2(np.fft.ifftshift(kern)))
+    img += texture * smooth / np.abs(smooth).max()
+    return img.astype(np.complex64)
+
+
+def stack_of_spirals(npartitions: int, ninterleaves: int, nsamples: int, density_power: float = 1.5):
+    """3-D stack of spirals: ``npartitions`` k_z planes in [-pi, pi), each
+    with ``ninterleaves`` variable-density spiral interleaves in (k_y, k_x);
+    float32 arrays (axis 0, 1, 2)."""
+    ky, kx = spiral2d(ninterleaves, nsamples, density_power)
+    kz_planes = -np.pi + 2 * np.pi * np.arange(npartitions) / npartitions
+    kz = np.repeat(kz_planes, ky.size).astype(np.float32)
+    return [kz, np.tile(ky, npartitions), np.tile(kx, npartitions)]

@@ -159,7 +159,7 @@ def test_sense_cg_implicit_gradient_vs_dense_fp64():
     y = crandn(rng, (C, M))
     op = ag.NufftOperator(shape, ntransf=C, tol=1e-6)
     k = [cuda(v).requires_grad_(True) for v in coords]
-    xh = ag.sense_cg_solve(cuda(y), cuda(S), op, *k, iters=150)
+    xh = ag.sense_cg_solve(cuda(y), cuda(S), op, *k, iters=150, method="implicit")
     loss = (xh - cuda(x_true)).abs().pow(2).sum()
     loss.backward()
     # dense FP64 reference: x_hat = (sum_c A_c^H A_c)^{-1} sum_c A_c^H y_c with A_c = E diag(S_c)
@@ -175,3 +175,48 @@ def test_sense_cg_implicit_gradient_vs_dense_fp64():
     report("CG solution vs dense FP64 solve", rel_l2(host(xh).reshape(-1), x64.detach().numpy()), 1e-4)
     for a in range(2):
         report(f"CG-solve d loss / d k[{a}] vs dense FP64", rel_l2(host(k[a].grad), kk[a].grad.numpy()), 1e-4)
+
+
+def test_truncated_cg_adjoint_gradient_vs_fp64_unrolled():
+    """10 CG steps (not converged): the reverse-mode adjoint through the
+    checkpointed iterates against torch FP64 autograd through the same 10
+    dense-matrix CG steps."""
+    from hastycucompute_amd import autograd as ag
+    shape, C, M, iters = (16, 16), 3, 500, 10
+    rng = np.random.default_rng(16)
+    coords = rand_coords(rng, 2, M, -3.0, 3.0)
+    S = crandn(rng, (C,) + shape) * 0.6
+    x_true = crandn(rng, shape)
+    y = crandn(rng, (C, M))
+    op = ag.NufftOperator(shape, ntransf=C, tol=1e-6)
+    k = [cuda(v).requires_grad_(True) for v in coords]
+    yg = cuda(y).requires_grad_(True)
+    xh = ag.sense_cg_solve(yg, cuda(S), op, *k, iters=iters)
+    loss = (xh - cuda(x_true)).abs().pow(2).sum()
+    loss.backward()
+    kk = [torch.from_numpy(v.astype(np.float64)).requires_grad_(True) for v in coords]
+    y64 = torch.from_numpy(y.astype(np.complex128)).requires_grad_(True)
+    S64 = torch.from_numpy(S.astype(np.complex128)).reshape(C, -1)
+    E = _ndft_matrix(kk, shape)
+    V = float(np.prod(shape))
+    Ac = [E * S64[c][None, :] for c in range(C)]
+    Nmat = sum(a.conj().T @ a for a in Ac) / V
+    b = sum(Ac[c].conj().T @ y64[c] for c in range(C)) / V
+    x = torch.zeros_like(b)
+    r = b.clone()
+    p = r.clone()
+    rz = torch.vdot(r, r).real
+    for _ in range(iters):
+        q = Nmat @ p
+        alpha = rz / torch.vdot(p, q).real
+        x = x + alpha * p
+        r = r - alpha * q
+        rz1 = torch.vdot(r, r).real
+        p = r + (rz1 / rz) * p
+        rz = rz1
+    L = (x - torch.from_numpy(x_true.astype(np.complex128)).reshape(-1)).abs().pow(2).sum()
+    L.backward()
+    report(f"{iters}-step CG iterate vs FP64 unrolled CG", rel_l2(host(xh).reshape(-1), x.detach().numpy()), 1e-4)
+    report("truncated-CG d loss / d y vs FP64 autograd", rel_l2(host(yg.grad), y64.grad.numpy()), 1e-4)
+    for a in range(2):
+        report(f"truncated-CG d loss / d k[{a}] vs FP64 autograd", rel_l2(host(k[a].grad), kk[a].grad.numpy()), 1e-4)
