@@ -215,37 +215,86 @@ __global__ void fold_count_kernel(FoldArgs<CT> a, int* ccount, int* vcount) {
                           1);
 }
 
-template <class CT, class R>
-__global__ void fold_scatter_kernel(FoldArgs<CT> a, int* ccursor, int* vcursor, int* sidx,
-                                    int* sinv, int* sl0, R* sfrac, R* sw, int w, int ws, double beta,
-                                    int4* visits) {
+// Sort keys of the points in caller order (cell key, then the point index as
+// the value of a stable radix sort: the sorted order is the same on every
+// call for the same trajectory).
+template <class CT>
+__global__ void fold_key_kernel(FoldArgs<CT> a, unsigned* keys, int* idx) {
     const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (j >= a.M) return;
     int l0w[3], t3[3];
     double fr[3];
     fold_point(a, j, l0w, fr, t3);
-    const int pos = atomicAdd(&ccursor[cell_key(a.g, l0w, t3)], 1);
-    sidx[pos] = int(j);
-    sinv[j] = pos;
+    keys[j] = unsigned(cell_key(a.g, l0w, t3));
+    idx[j] = int(j);
+}
+
+// Per sorted point s: support start, fractional offset, window weights
+// (FP64 evaluation); per original point j the number of tiles its support
+// touches.
+template <class CT, class R>
+__global__ void fold_sorted_kernel(FoldArgs<CT> a, const int* sidx, int* sinv, int* sl0, R* sfrac, R* sw,
+                                   int w, int ws, double beta, int* vcnt) {
+    const int64_t pos = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (pos >= a.M) return;
+    const int64_t j = sidx[pos];
+    sinv[j] = int(pos);
+    int l0w[3], t3[3];
+    double fr[3];
+    fold_point(a, j, l0w, fr, t3);
     const double ihw = 2.0 / double(w);
     for (int s = 0; s < 3; ++s) {
         sl0[int64_t(s) * a.M + pos] = l0w[s];
         sfrac[int64_t(s) * a.M + pos] = R(fr[s]);
-        R* wp = sw + int64_t(pos) * ws + s * w;
+        R* wp = sw + pos * ws + s * w;
         for (int q = 0; q < w; ++q)
             wp[q] = a.g.wa[s] == 1 ? R(q == 0 ? 1.0 : 0.0) : R(kb_phi<double>(fr[s] - double(q), ihw, beta));
     }
-    for (int q = 3 * w; q < ws; ++q) sw[int64_t(pos) * ws + q] = R(0);
-    const int e0 = second_tile(a.g, 0, l0w[0]), e1 = second_tile(a.g, 1, l0w[1]),
-              e2 = second_tile(a.g, 2, l0w[2]);
+    for (int q = 3 * w; q < ws; ++q) sw[pos * ws + q] = R(0);
+    vcnt[j] = (1 + second_tile(a.g, 0, l0w[0])) * (1 + second_tile(a.g, 1, l0w[1])) *
+              (1 + second_tile(a.g, 2, l0w[2]));
+}
+
+// The (tile, 8 s + neighbour bits) pair of every visit, in the caller's point
+// order (vpos = exclusive scan of vcnt over j); a stable sort by tile then
+// groups them, keeping that order inside each tile (consecutive visits of a
+// tile then spread along the acquisition, e.g. along spokes, which balances
+// the sub-blocks of a tile within each chunk of visits).
+__global__ void visit_emit_kernel(const int* sl0, const int* sinv, int64_t M, Geo g, const int* vpos,
+                                  unsigned* vkey, int* vval) {
+    const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= M) return;
+    const int64_t s = sinv[j];
+    int l0w[3], t3[3];
+    for (int d = 0; d < 3; ++d) {
+        l0w[d] = sl0[int64_t(d) * M + s];
+        t3[d] = l0w[d] / g.T[d];
+    }
+    const int e0 = second_tile(g, 0, l0w[0]), e1 = second_tile(g, 1, l0w[1]), e2 = second_tile(g, 2, l0w[2]);
+    int v = vpos[j];
     for (int bz = 0; bz <= e0; ++bz)
         for (int by = 0; by <= e1; ++by)
             for (int bx = 0; bx <= e2; ++bx) {
-                const int t = tile_id(a.g, (t3[0] + bz) % a.g.ntile[0],
-                                      (t3[1] + by) % a.g.ntile[1], (t3[2] + bx) % a.g.ntile[2]);
-                const int v = atomicAdd(&vcursor[t], 1);
-                visits[v] = make_int4(pos * 8 + (bz << 2 | by << 1 | bx), l0w[0], l0w[1], l0w[2]);
+                vkey[v] = unsigned(tile_id(g, (t3[0] + bz) % g.ntile[0], (t3[1] + by) % g.ntile[1],
+                                           (t3[2] + bx) % g.ntile[2]));
+                vval[v] = int(s) * 8 + (bz << 2 | by << 1 | bx);
+                ++v;
             }
+}
+
+__global__ void visit_records_kernel(const int* vval, const int* sl0, int64_t M, int64_t nvis, int4* visits) {
+    const int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (v >= nvis) return;
+    const int val = vval[v];
+    const int64_t s = val >> 3;
+    visits[v] = make_int4(val, sl0[s], sl0[M + s], sl0[2 * M + s]);
+}
+
+// Significant bits of keys below n.
+inline int key_bits(int64_t n) {
+    int b = 1;
+    while ((int64_t(1) << b) < n) ++b;
+    return b;
 }
 
 // toff[t] = first sorted point of tile t (t <= ntiles) from the cell offsets.
@@ -464,9 +513,10 @@ void nufft_setpts_typed(NufftPlan* p, int64_t M, const void* c0, const void* c1,
     const int64_t nt = p->ntiles;
     const int64_t cpt = int64_t(p->T[0]) * p->T[1] * p->T[2];
     const int64_t nkeys = nt * cpt;                      // = fine-grid cells
+    HASTY_REQUIRE(nkeys <= (int64_t(1) << 32), "fine grid too large for one plan (2^32 cells)");
     int64_t tally_tmp = 0;
-    // ccount[nkeys] -> (scan) coff[nkeys] used as the scatter cursor;
-    // vcount / vcursor [nt]; toff [nt + 1]
+    // ccount[nkeys] -> (scan) coff[nkeys] -> tile offsets toff [nt + 1];
+    // vcount [nt] (visits per tile)
     int* ccount = dev_alloc<int>(nkeys, tally_tmp);
     int* coff = dev_alloc<int>(nkeys, tally_tmp);
     int* vbuf = dev_alloc<int>(2 * nt, tally_tmp);
@@ -509,13 +559,11 @@ void nufft_setpts_typed(NufftPlan* p, int64_t M, const void* c0, const void* c1,
     HASTY_CHECK_HIP(hipMemcpyAsync(hvcount.data(), vbuf, size_t(nt) * sizeof(int), hipMemcpyDeviceToHost, s));
     HASTY_CHECK_HIP(hipStreamSynchronize(s));
     // subproblems; split tiles get one partial slot per subproblem
-    std::vector<int> vcursors(static_cast<size_t>(nt));
     std::vector<int4> isubs, ssubs, splits;
     int64_t voff = 0;
     int nslots = 0;
     for (int64_t t = 0; t < nt; ++t) {
         const int poff = htoff[t], pc = htoff[t + 1] - htoff[t], vc = hvcount[t];
-        vcursors[t] = int(voff);
         for (int st = 0; st < pc; st += p->max_sub)
             isubs.push_back(make_int4(int(t), poff + st, std::min(p->max_sub, pc - st), 0));
         if (vc <= p->max_sub) {
@@ -549,23 +597,72 @@ void nufft_setpts_typed(NufftPlan* p, int64_t M, const void* c0, const void* c1,
     upload_int4(isubs, p->sub_interp, p->sub_interp_cap, p->nsub_interp);
     upload_int4(ssubs, p->sub_spread, p->sub_spread_cap, p->nsub_spread);
     upload_int4(splits, p->split_info, p->split_cap, p->nsplit);
-    HASTY_CHECK_HIP(hipMemcpyAsync(vbuf + nt, vcursors.data(), size_t(nt) * sizeof(int),
-                                   hipMemcpyHostToDevice, s));
     if (M > 0) {
-        auto scatter = [&](auto* ctag, auto* rtag) {
+        // 1. stable radix sort of (cell key, point index): the sorted order
+        unsigned* keys_in = dev_alloc<unsigned>(M, tally_tmp);
+        unsigned* keys_out = dev_alloc<unsigned>(M, tally_tmp);
+        int* idx_in = dev_alloc<int>(M, tally_tmp);
+        int* vcnt = dev_alloc<int>(M, tally_tmp);
+        int* vpos = dev_alloc<int>(M, tally_tmp);
+        if (coords_f64) {
+            auto fa = run_fold((double*)nullptr);
+            hipLaunchKernelGGL(fold_key_kernel<double>, dim3(blocks), dim3(256), 0, s, fa, keys_in, idx_in);
+        } else {
+            auto fa = run_fold((float*)nullptr);
+            hipLaunchKernelGGL(fold_key_kernel<float>, dim3(blocks), dim3(256), 0, s, fa, keys_in, idx_in);
+        }
+        HASTY_CHECK_HIP(hipGetLastError());
+        auto sort_pairs = [&](const unsigned* kin, unsigned* kout, const int* vin, int* vout, int64_t n, int bits) {
+            size_t tb = 0;
+            HASTY_CHECK_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, kin, kout, vin, vout, int(n), 0, bits, s));
+            void* tmp = dev_alloc<unsigned char>(int64_t(std::max<size_t>(tb, 1)), tally_tmp);
+            HASTY_CHECK_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, kin, kout, vin, vout, int(n), 0, bits, s));
+            HASTY_CHECK_HIP(hipStreamSynchronize(s));
+            HASTY_CHECK_HIP(hipFree(tmp));
+        };
+        sort_pairs(keys_in, keys_out, idx_in, p->sidx, M, key_bits(nkeys));
+        // 2. per sorted point: support, weights, visit count
+        auto fold_sorted = [&](auto* ctag, auto* rtag) {
             using CT = std::remove_pointer_t<decltype(ctag)>;
             using RT = std::remove_pointer_t<decltype(rtag)>;
             auto fa = run_fold((CT*)nullptr);
-            hipLaunchKernelGGL((fold_scatter_kernel<CT, RT>), dim3(blocks), dim3(256), 0, s, fa,
-                               coff, vbuf + nt, p->sidx, p->sinv, p->sl0,
+            hipLaunchKernelGGL((fold_sorted_kernel<CT, RT>), dim3(blocks), dim3(256), 0, s, fa,
+                               static_cast<const int*>(p->sidx), p->sinv, p->sl0,
                                static_cast<RT*>(p->sfrac), static_cast<RT*>(p->sw), p->w,
-                               weight_stride(p->w), p->beta, p->visits);
+                               weight_stride(p->w), p->beta, vcnt);
         };
-        if (coords_f64 && p->f64) scatter((double*)nullptr, (double*)nullptr);
-        else if (coords_f64) scatter((double*)nullptr, (float*)nullptr);
-        else if (p->f64) scatter((float*)nullptr, (double*)nullptr);
-        else scatter((float*)nullptr, (float*)nullptr);
+        if (coords_f64 && p->f64) fold_sorted((double*)nullptr, (double*)nullptr);
+        else if (coords_f64) fold_sorted((double*)nullptr, (float*)nullptr);
+        else if (p->f64) fold_sorted((float*)nullptr, (double*)nullptr);
+        else fold_sorted((float*)nullptr, (float*)nullptr);
         HASTY_CHECK_HIP(hipGetLastError());
+        // 3. visits in sorted point order, then a stable sort by tile
+        {
+            size_t tb = 0;
+            HASTY_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, vcnt, vpos, M, s));
+            void* tmp = dev_alloc<unsigned char>(int64_t(std::max<size_t>(tb, 1)), tally_tmp);
+            HASTY_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tb, vcnt, vpos, M, s));
+            HASTY_CHECK_HIP(hipStreamSynchronize(s));
+            HASTY_CHECK_HIP(hipFree(tmp));
+        }
+        unsigned* vkey_in = dev_alloc<unsigned>(voff, tally_tmp);
+        unsigned* vkey_out = dev_alloc<unsigned>(voff, tally_tmp);
+        int* vval_in = dev_alloc<int>(voff, tally_tmp);
+        int* vval_out = dev_alloc<int>(voff, tally_tmp);
+        hipLaunchKernelGGL(visit_emit_kernel, dim3(blocks), dim3(256), 0, s, static_cast<const int*>(p->sl0),
+                           static_cast<const int*>(p->sinv), M, g, static_cast<const int*>(vpos), vkey_in,
+                           vval_in);
+        HASTY_CHECK_HIP(hipGetLastError());
+        sort_pairs(vkey_in, vkey_out, vval_in, vval_out, voff, key_bits(nt));
+        hipLaunchKernelGGL(visit_records_kernel, dim3(unsigned(ceil_div(voff, 256))), dim3(256), 0, s,
+                           static_cast<const int*>(vval_out), static_cast<const int*>(p->sl0), M, voff,
+                           p->visits);
+        HASTY_CHECK_HIP(hipGetLastError());
+        HASTY_CHECK_HIP(hipStreamSynchronize(s));
+        for (void* q : {static_cast<void*>(keys_in), static_cast<void*>(keys_out), static_cast<void*>(idx_in),
+                        static_cast<void*>(vcnt), static_cast<void*>(vpos), static_cast<void*>(vkey_in),
+                        static_cast<void*>(vkey_out), static_cast<void*>(vval_in), static_cast<void*>(vval_out)})
+            HASTY_CHECK_HIP(hipFree(q));
     }
     // the host vectors above are copied asynchronously: wait before they go
     HASTY_CHECK_HIP(hipStreamSynchronize(s));

@@ -201,3 +201,33 @@ def test_many_coil_kooshball_vs_kb_oracle(ntransf, kw):
     kb = KBPlan(shape, EPS).setpts(coords)
     report(f"kooshball C={ntransf} {kw} type2 vs KB oracle", rel_l2(g2, kb.type2(f)), 2e-6)
     report(f"kooshball C={ntransf} {kw} type1 vs KB oracle", rel_l2(g1, kb.type1(c)), 2e-6)
+
+
+def test_setpts_is_deterministic():
+    """The sorted point order and the visit lists come from stable sorts, so
+    the same trajectory gives bitwise the same transforms on every setpts."""
+    from hastycucompute_amd import synth
+    shape, C = (20, 24, 16), 4
+    k = [cuda(c) for c in synth.kooshball(400, 40)]
+    M = int(k[0].shape[0])
+    rng = np.random.default_rng(31)
+    c = cuda(crandn(rng, (C, M)))
+    f = cuda(crandn(rng, (C,) + shape))
+    outs1, outs2 = [], []
+    for _ in range(3):
+        p1, p2 = _plans(shape, C)
+        p1.setpts(k)
+        p2.setpts(k)
+        o1 = torch.empty((C,) + shape, dtype=torch.complex64, device="cuda")
+        o2 = torch.empty((C, M), dtype=torch.complex64, device="cuda")
+        p1.execute(c, o1)
+        p2.execute(f, o2)
+        p1.setpts(k)            # and again on the same plan
+        o1b = torch.empty_like(o1)
+        p1.execute(c, o1b)
+        assert torch.equal(o1, o1b)
+        outs1.append(o1)
+        outs2.append(o2)
+    assert all(torch.equal(outs1[0], o) for o in outs1[1:])
+    assert all(torch.equal(outs2[0], o) for o in outs2[1:])
+    print("[parity] setpts: bitwise identical type 1 / type 2 over repeated setpts and plans", flush=True)

@@ -189,7 +189,7 @@ def test_cg_gradient_160c16_stack_of_spirals():
     # central differences of the same pipeline along 8 random unit directions
     # in trajectory space (h = 1 moves each point by ~4e-4 rad on average)
     gen = torch.Generator(device=dev).manual_seed(8)
-    errs = []
+    pairs = []
     for _ in range(8):
         d = [torch.randn(c.shape, device=dev, dtype=torch.float64, generator=gen) for c in k]
         nrm = float(torch.sqrt(sum((di ** 2).sum() for di in d)))
@@ -200,7 +200,10 @@ def test_cg_gradient_160c16_stack_of_spirals():
             lm = float(loss_of([(c.double() - h * di).float() for c, di in zip(k, d)]))
         fd = (lp - lm) / (2 * h)
         an = float(sum((gi * di).sum() for gi, di in zip(g, d)))
-        errs.append(abs(an - fd) / max(abs(fd), 1e-30))
-        print(f"[workload]   direction: adjoint {an:+.6e}  central difference {fd:+.6e}  "
-              f"rel {errs[-1]:.2e}", flush=True)
-    report("160^3 truncated-CG gradient vs central differences (8 random directions, max rel)", max(errs), 2e-2)
+        pairs.append((an, fd))
+        print(f"[workload]   direction: adjoint {an:+.6e}  central difference {fd:+.6e}", flush=True)
+    # errors relative to the typical directional derivative (a direction
+    # whose derivative happens to be near zero is judged on the same scale)
+    scale = float(np.sqrt(np.mean([fd * fd for _, fd in pairs])))
+    report("160^3 truncated-CG gradient vs central differences (8 random directions, max |err| / rms)",
+           max(abs(an - fd) for an, fd in pairs) / scale, 2e-2)
