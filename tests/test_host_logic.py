@@ -72,3 +72,46 @@ def test_coil_maps_rss_normalised():
 def test_spiral_in_range():
     ky, kx = synth.spiral2d(4, 256)
     assert np.all(np.abs(ky) <= np.pi) and np.all(np.abs(kx) <= np.pi)
+
+
+def test_stack_of_spirals_geometry():
+    kz, ky, kx = synth.stack_of_spirals(40, 24, 2048)
+    assert kz.size == ky.size == kx.size == 40 * 24 * 2048
+    planes = np.unique(kz)
+    assert planes.size == 40 and np.allclose(np.diff(planes), 2 * np.pi / 40, atol=1e-5)
+    assert np.all(np.abs(ky) <= np.pi) and np.all(np.abs(kx) <= np.pi)
+    # every partition carries the same in-plane spiral set
+    assert np.array_equal(ky[:24 * 2048], ky[24 * 2048:2 * 24 * 2048])
+
+
+def test_ellipse_phantom_is_deterministic():
+    a = synth.ellipse_phantom((64, 48), seed=3)
+    b = synth.ellipse_phantom((64, 48), seed=3)
+    assert a.dtype == np.complex64 and a.shape == (64, 48) and np.array_equal(a, b)
+    assert np.abs(a).max() > 0.1
+
+
+def test_trajectory_penalty_cpu():
+    from hastycucompute_amd import trajectory
+    # a straight line sampled uniformly has zero slew; a step in speed has
+    # non-zero slew, and speeds above gmax are penalised
+    t = torch.linspace(-1, 1, 64, dtype=torch.float64)
+    k = [t.clone(), torch.zeros_like(t)]
+    assert float(trajectory.penalty(k, 1, gmax=1.0, slew_weight=1.0, grad_weight=1.0)) < 1e-20
+    bent = [torch.cat([t[:32], t[32:] * 2]), torch.zeros_like(t)]
+    assert float(trajectory.penalty(bent, 1, gmax=1.0, slew_weight=1.0, grad_weight=0.0)) > 0
+    fast = [t * 100, torch.zeros_like(t)]
+    assert float(trajectory.penalty(fast, 1, gmax=0.5, slew_weight=0.0, grad_weight=1.0)) > 0
+
+
+def test_autotune_candidates_and_options():
+    from hastycucompute_amd import autotune
+    for nt in (1, 2):
+        for dim in (2, 3):
+            c = autotune._candidates(nt, dim)
+            assert c and all(len(x["tile"]) == 3 for x in c)
+            if dim == 2:
+                assert all(x["tile"][0] == 1 for x in c)
+    opts = autotune.plan_options({"tile": [8, 8, 16], "max_subproblem": 1024, "ms": 1.0, "trials": []})
+    assert opts == {"tile": (8, 8, 16), "max_subproblem": 1024}
+    hf.nufft_opts(nmodes=(8, 8, 8), **opts)      # accepted by the options block
