@@ -389,6 +389,19 @@ void* kernel_ptr(const NufftPlan* p, int which) {
     return p->f64 ? nufft_kernel_ptr<double2>(which, p->w) : nufft_kernel_ptr<float2>(which, p->w);
 }
 
+// The interpolation kernel for this plan: the compile-time-tile variant when
+// the plan uses the standard tile (16^3 in 3-D, 32^2 in 2-D).
+void* interp_ptr(const NufftPlan* p) {
+    const bool std3 = p->dim == 3 && p->T[0] == 16 && p->T[1] == 16 && p->T[2] == 16;
+    const bool std2 = p->dim == 2 && p->T[0] == 1 && p->T[1] == 32 && p->T[2] == 32;
+    if (std3 || std2) {
+        void* fn = p->f64 ? nufft_interp_fixed_ptr<double2>(p->dim, p->w)
+                          : nufft_interp_fixed_ptr<float2>(p->dim, p->w);
+        if (fn) return fn;
+    }
+    return kernel_ptr(p, 0);
+}
+
 }  // namespace
 
 NufftPlan* nufft_makeplan(int type, int dim, const int64_t* nmodes, int sign, int ntransf,
@@ -452,7 +465,7 @@ NufftPlan* nufft_makeplan(int type, int dim, const int64_t* nmodes, int sign, in
     }
     p->max_batch = (opts && opts->max_batch > 0) ? std::min(opts->max_batch, ntransf) : ntransf;
     p->max_sub = (opts && opts->max_subproblem > 0) ? opts->max_subproblem : (type == 2 ? 8192 : 2048);
-    if (!p->gm && type == 2) set_lds_limit(kernel_ptr(p.get(), 0), interp_lds_bytes(p.get(), p->group));
+    if (!p->gm && type == 2) set_lds_limit(interp_ptr(p.get()), interp_lds_bytes(p.get(), p->group));
     // deapodisation factors 1/Phi(2 pi m / nf) per slot, centred modes
     for (int s = 0; s < 3; ++s) {
         const int64_t n = p->n[s];
@@ -740,10 +753,14 @@ void execute_impl(NufftPlan* p, V* c, V* f) {
                                    dim3(unsigned(ceil_div(p->M, 256))), dim3(256), 0, s, ga);
             } else {
                 ga.subs = p->sub_interp;
+                ga.nsub = p->nsub_interp;
+                ga.ngroups = int(ngroups);
                 HASTY_PROF("nufft_interp", s);
-                hipLaunchKernelGGL(reinterpret_cast<GridFn<V>>(kernel_ptr(p, 0)),
-                                   dim3(unsigned(p->nsub_interp), ngroups), dim3(kNufftThreads),
-                                   interp_lds_bytes(p, ga.group), s, ga);
+                // XCD-aware numbering (xcd_block): a 1-D grid of
+                // ceil(nsub / 8) * 8 * ngroups ids
+                const int64_t nblocks = ceil_div(p->nsub_interp, kXcds) * kXcds * int64_t(ngroups);
+                hipLaunchKernelGGL(reinterpret_cast<GridFn<V>>(interp_ptr(p)), dim3(unsigned(nblocks)),
+                                   dim3(kNufftThreads), interp_lds_bytes(p, ga.group), s, ga);
             }
             HASTY_CHECK_HIP(hipGetLastError());
         } else {

@@ -30,4 +30,31 @@ template <> void* nufft_spread_ptr<float2>(int cg) {
     throw Error(HASTY_ERR_INTERNAL, "no spreading kernel for this transform group");
 }
 
+template <> void* nufft_interp_fixed_ptr<float2>(int dim, int w) {
+    if (dim == 3) {
+        switch (w) {
+            case 2: return reinterpret_cast<void*>(&interp_kernel<float2, 2, 16, 16, 16>);
+            case 3: return reinterpret_cast<void*>(&interp_kernel<float2, 3, 16, 16, 16>);
+            case 4: return reinterpret_cast<void*>(&interp_kernel<float2, 4, 16, 16, 16>);
+            case 5: return reinterpret_cast<void*>(&interp_kernel<float2, 5, 16, 16, 16>);
+            case 6: return reinterpret_cast<void*>(&interp_kernel<float2, 6, 16, 16, 16>);
+            case 7: return reinterpret_cast<void*>(&interp_kernel<float2, 7, 16, 16, 16>);
+            case 8: return reinterpret_cast<void*>(&interp_kernel<float2, 8, 16, 16, 16>);
+            case 9: return reinterpret_cast<void*>(&interp_kernel<float2, 9, 16, 16, 16>);
+        }
+    } else {
+        switch (w) {
+            case 2: return reinterpret_cast<void*>(&interp_kernel<float2, 2, 1, 32, 32>);
+            case 3: return reinterpret_cast<void*>(&interp_kernel<float2, 3, 1, 32, 32>);
+            case 4: return reinterpret_cast<void*>(&interp_kernel<float2, 4, 1, 32, 32>);
+            case 5: return reinterpret_cast<void*>(&interp_kernel<float2, 5, 1, 32, 32>);
+            case 6: return reinterpret_cast<void*>(&interp_kernel<float2, 6, 1, 32, 32>);
+            case 7: return reinterpret_cast<void*>(&interp_kernel<float2, 7, 1, 32, 32>);
+            case 8: return reinterpret_cast<void*>(&interp_kernel<float2, 8, 1, 32, 32>);
+            case 9: return reinterpret_cast<void*>(&interp_kernel<float2, 9, 1, 32, 32>);
+        }
+    }
+    return nullptr;
+}
+
 }  // namespace hasty

@@ -30,4 +30,41 @@ template <> void* nufft_spread_ptr<double2>(int cg) {
     throw Error(HASTY_ERR_INTERNAL, "no spreading kernel for this transform group");
 }
 
+template <> void* nufft_interp_fixed_ptr<double2>(int dim, int w) {
+    if (dim == 3) {
+        switch (w) {
+            case 2: return reinterpret_cast<void*>(&interp_kernel<double2, 2, 16, 16, 16>);
+            case 3: return reinterpret_cast<void*>(&interp_kernel<double2, 3, 16, 16, 16>);
+            case 4: return reinterpret_cast<void*>(&interp_kernel<double2, 4, 16, 16, 16>);
+            case 5: return reinterpret_cast<void*>(&interp_kernel<double2, 5, 16, 16, 16>);
+            case 6: return reinterpret_cast<void*>(&interp_kernel<double2, 6, 16, 16, 16>);
+            case 7: return reinterpret_cast<void*>(&interp_kernel<double2, 7, 16, 16, 16>);
+            case 8: return reinterpret_cast<void*>(&interp_kernel<double2, 8, 16, 16, 16>);
+            case 9: return reinterpret_cast<void*>(&interp_kernel<double2, 9, 16, 16, 16>);
+            case 10: return reinterpret_cast<void*>(&interp_kernel<double2, 10, 16, 16, 16>);
+            case 11: return reinterpret_cast<void*>(&interp_kernel<double2, 11, 16, 16, 16>);
+            case 12: return reinterpret_cast<void*>(&interp_kernel<double2, 12, 16, 16, 16>);
+            case 13: return reinterpret_cast<void*>(&interp_kernel<double2, 13, 16, 16, 16>);
+            case 14: return reinterpret_cast<void*>(&interp_kernel<double2, 14, 16, 16, 16>);
+        }
+    } else {
+        switch (w) {
+            case 2: return reinterpret_cast<void*>(&interp_kernel<double2, 2, 1, 32, 32>);
+            case 3: return reinterpret_cast<void*>(&interp_kernel<double2, 3, 1, 32, 32>);
+            case 4: return reinterpret_cast<void*>(&interp_kernel<double2, 4, 1, 32, 32>);
+            case 5: return reinterpret_cast<void*>(&interp_kernel<double2, 5, 1, 32, 32>);
+            case 6: return reinterpret_cast<void*>(&interp_kernel<double2, 6, 1, 32, 32>);
+            case 7: return reinterpret_cast<void*>(&interp_kernel<double2, 7, 1, 32, 32>);
+            case 8: return reinterpret_cast<void*>(&interp_kernel<double2, 8, 1, 32, 32>);
+            case 9: return reinterpret_cast<void*>(&interp_kernel<double2, 9, 1, 32, 32>);
+            case 10: return reinterpret_cast<void*>(&interp_kernel<double2, 10, 1, 32, 32>);
+            case 11: return reinterpret_cast<void*>(&interp_kernel<double2, 11, 1, 32, 32>);
+            case 12: return reinterpret_cast<void*>(&interp_kernel<double2, 12, 1, 32, 32>);
+            case 13: return reinterpret_cast<void*>(&interp_kernel<double2, 13, 1, 32, 32>);
+            case 14: return reinterpret_cast<void*>(&interp_kernel<double2, 14, 1, 32, 32>);
+        }
+    }
+    return nullptr;
+}
+
 }  // namespace hasty

@@ -45,27 +45,51 @@ struct GridArgs {
     int64_t M;
     int nbatch, group, w, ws;
     V* partial;             // [slot][nbatch][tile cells]: partial tiles of split tiles
+    int nsub, ngroups;      // interpolation: subproblems and transform groups (xcd_block)
     Geo g;
 };
+
+// Interpolation blocks are numbered so that the transform groups of one
+// subproblem are consecutive in dispatch order on one XCD: workgroups go to
+// the 8 XCDs round-robin by id, so with id = ((sub / 8) * ngroups + group) * 8
+// + sub % 8 every group of `sub` lands on XCD sub % 8, one after the other,
+// and the subproblem's point data (weights, indices) is read from HBM about
+// once and from that XCD's L2 for the other groups.  Returns false for the
+// padding ids past the last subproblem.
+constexpr int kXcds = 8;
+__device__ inline bool xcd_block(int nsub, int ngroups, int& sub, int& group) {
+    const int id = int(blockIdx.x);
+    const int r = id % kXcds, q = id / kXcds;
+    group = q % ngroups;
+    sub = (q / ngroups) * kXcds + r;
+    return sub < nsub;
+}
 
 // ---------------------------------------------------------------------------
 // Interpolation: one block per (tile subproblem, transform group); the padded
 // tile footprint [t0, t0 + T + w - 1) of each transform is staged in LDS and
-// each thread gathers the w^d cells of its points.
+// each thread gathers the w^d cells of its points.  TZ/TY/TX > 0 fix the tile
+// edge at compile time (the standard 16^3 / 32^2 tiles): the padded pitch is
+// then a constant and every cell read is an immediate offset from one
+// per-point base address; 0 reads the tile from the plan.
 // ---------------------------------------------------------------------------
-template <class V, int W>
+template <class V, int W, int TZ = 0, int TY = 0, int TX = 0>
 __global__ void __launch_bounds__(kNufftThreads) interp_kernel(GridArgs<V> a) {
     using R = decltype(V::x);
     extern __shared__ unsigned char smem_raw[];
     V* lds = reinterpret_cast<V*>(smem_raw);
-    const int4 sub = a.subs[blockIdx.x];
+    int isub, igroup;
+    if (!xcd_block(a.nsub, a.ngroups, isub, igroup)) return;
+    const int4 sub = a.subs[isub];
     int t0[3];
     tile_origin(a.g, sub.x, t0);
-    const int Pz = a.g.T[0] + a.g.wa[0] - 1;
-    const int Py = a.g.T[1] + a.g.wa[1] - 1;
-    const int Px = a.g.T[2] + a.g.wa[2] - 1;
+    constexpr bool FIXED = TX > 0;
+    const int wz = FIXED ? (TZ == 1 ? 1 : W) : a.g.wa[0];
+    const int Pz = FIXED ? (TZ == 1 ? 1 : TZ + W - 1) : a.g.T[0] + a.g.wa[0] - 1;
+    const int Py = FIXED ? TY + W - 1 : a.g.T[1] + a.g.wa[1] - 1;
+    const int Px = FIXED ? TX + W - 1 : a.g.T[2] + a.g.wa[2] - 1;
     const int cells = Pz * Py * Px;
-    const int g0 = blockIdx.y * a.group;
+    const int g0 = igroup * a.group;
     const int ng = min(a.group, a.nbatch - g0);
     const int64_t G = a.g.nf[0] * a.g.nf[1] * a.g.nf[2];
     // stage, 4 loads in flight per thread
@@ -94,7 +118,6 @@ __global__ void __launch_bounds__(kNufftThreads) interp_kernel(GridArgs<V> a) {
         }
     }
     __syncthreads();
-    const int wz = a.g.wa[0];
     for (int s = sub.y + threadIdx.x; s < sub.y + sub.z; s += blockDim.x) {
         const int lz = a.sl0[s] - t0[0];
         const int ly = a.sl0[a.M + s] - t0[1];
@@ -108,21 +131,25 @@ __global__ void __launch_bounds__(kNufftThreads) interp_kernel(GridArgs<V> a) {
             wx[q] = w[2 * W + q];
         }
         const int orig = a.sidx[s];
+        const V* pbase = lds + (lz * Py + ly) * Px + lx;
         for (int gi = 0; gi < ng; ++gi) {
+            const V* tb = pbase + gi * cells;
             V acc = cmake<V>(R(0), R(0));
 #pragma unroll
             for (int qz = 0; qz < W; ++qz) {
                 if (qz < wz) {
-                    const V* base = lds + gi * cells + ((lz + qz) * Py + ly) * Px + lx;
                     V accy = cmake<V>(R(0), R(0));
 #pragma unroll
                     for (int qy = 0; qy < W; ++qy) {
+                        // the whole row first: its LDS reads are in flight together
+                        V row[W];
+#pragma unroll
+                        for (int qx = 0; qx < W; ++qx) row[qx] = tb[(qz * Py + qy) * Px + qx];
                         V accx = cmake<V>(R(0), R(0));
 #pragma unroll
                         for (int qx = 0; qx < W; ++qx) {
-                            const V v = base[qy * Px + qx];
-                            accx.x += wx[qx] * v.x;
-                            accx.y += wx[qx] * v.y;
+                            accx.x += wx[qx] * row[qx].x;
+                            accx.y += wx[qx] * row[qx].y;
                         }
                         accy.x += wy[qy] * accx.x;
                         accy.y += wy[qy] * accx.y;
@@ -142,9 +169,9 @@ __global__ void __launch_bounds__(kNufftThreads) interp_kernel(GridArgs<V> a) {
 // (4 x 4 x 4 in 3-D, 1 x 8 x 8 in 2-D) and each lane one cell of it, with
 // the CG complex accumulators of its cell in registers (FP32 for complex64).
 // The wave walks the tile's visit list (points whose support touches the
-// tile), skips with scalar tests the points whose support misses its
-// sub-block, and every lane adds w_z w_y w_x c over the CG transforms of the
-// point.  No atomics and no LDS; each cell's sum is taken in visit order, so
+// tile) 64 records at a time, one per lane; the lanes test their records
+// against the sub-block and the wave takes the hits in visit order (ballot),
+// every lane adding w_z w_y w_x c over the CG transforms of the point.  No atomics and no LDS; each cell's sum is taken in visit order, so
 // the result is the same on every run of a plan.  A split tile writes its
 // share to a partial slot (summed by reduce_split_tiles_kernel).
 // ---------------------------------------------------------------------------
@@ -164,8 +191,8 @@ __device__ inline bool arcs_meet(int a, int la, int b, int lb, int n) {
 }
 
 // The read-only arrays come as separate restrict-qualified arguments: with no
-// aliasing stores, the wave-uniform loads (visit records, the point's CG
-// values) become scalar loads.  csort holds CG - 1 padding elements past the
+// aliasing stores, the wave-uniform loads (the point's CG values) become
+// scalar loads.  csort holds CG - 1 padding elements past the
 // last point, so the CG values of a point are read without bounds tests.
 template <class V, int CG>
 __global__ void __launch_bounds__(kNufftThreads)
@@ -515,6 +542,9 @@ __global__ void __launch_bounds__(256) coord_grad_kernel(CoordGradArgs<V> a) {
 }
 
 template <class V> void* nufft_kernel_ptr(int which, int w);
+// interpolation kernel with the tile fixed at compile time (16^3 in 3-D,
+// 32^2 in 2-D), or nullptr when this width has none
+template <class V> void* nufft_interp_fixed_ptr(int dim, int w);
 // spreading kernel for CG transforms per block (1, 2, 4, 8, 16, 32 / 16 for complex128)
 template <class V> void* nufft_spread_ptr(int cg);
 
