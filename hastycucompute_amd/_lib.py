@@ -63,6 +63,12 @@ SIGNATURES = {
                                               c_vp, c_vp, P_i32, c_int, c_vp]),
     "hasty_toeplitz_scratch_bytes": (c_i64, [c_int, P_i64, c_i64]),
     "hasty_normal_create": (c_int, [c_int, P_i64, c_int, c_vp, c_vp, c_int, c_vp, ctypes.POINTER(c_vp)]),
+    "hasty_normal_idt_create": (c_int, [c_int, P_i64, c_int, c_vp, c_int, c_vp, c_vp, c_vp, c_int, c_vp,
+                                        ctypes.POINTER(c_vp)]),
+    "hasty_normal_t2_create": (c_int, [c_int, P_i64, c_int, c_vp, c_int, c_vp, c_vp, c_vp, c_int, c_vp,
+                                       ctypes.POINTER(c_vp)]),
+    "hasty_sense_forward": (c_int, [c_vp, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "hasty_sense_adjoint": (c_int, [c_vp, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_int]),
     "hasty_normal_apply": (c_int, [c_vp, c_vp, c_vp]),
     "hasty_normal_set_stream": (c_int, [c_vp, c_vp]),
     "hasty_normal_destroy": (c_int, [c_vp]),

@@ -121,6 +121,25 @@ int hasty_nufft_coord_grad(hasty_nufft_plan plan, const void* h, const void* w, 
     });
 }
 
+int hasty_sense_forward(hasty_nufft_plan plan, int ncoils, int L, const void* image,
+                        const void* smaps, const void* bil, const void* cjl, void* kdata) {
+    return guarded([&] {
+        HASTY_REQUIRE(plan != nullptr, "plan is NULL");
+        sense_offres_forward(reinterpret_cast<NufftPlan*>(plan), ncoils, L, image, smaps, bil, cjl,
+                             kdata);
+    });
+}
+
+int hasty_sense_adjoint(hasty_nufft_plan plan, int ncoils, int L, const void* kdata,
+                        const void* smaps, const void* bil, const void* cjl, void* images,
+                        int combine) {
+    return guarded([&] {
+        HASTY_REQUIRE(plan != nullptr, "plan is NULL");
+        sense_offres_adjoint(reinterpret_cast<NufftPlan*>(plan), ncoils, L, kdata, smaps, bil, cjl,
+                             images, combine ? 1 : 0);
+    });
+}
+
 int hasty_nufft_info(hasty_nufft_plan plan, int* width, double* beta, int64_t* nf3, int32_t* tile3,
                      int64_t* device_bytes) {
     return guarded([&] {

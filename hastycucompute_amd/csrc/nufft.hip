@@ -697,7 +697,7 @@ template <class V>
 using SpreadFn = void (*)(GridArgs<V>, const int4*, const V*, const decltype(V::x)*);
 
 template <class V>
-void execute_impl(NufftPlan* p, V* c, V* f) {
+void execute_impl(NufftPlan* p, V* c, V* f, const NufftHooks& hk, int ntransf) {
     using R = decltype(V::x);
     const hipStream_t s = p->stream;
     const Geo g = geo_of(p);
@@ -730,11 +730,27 @@ void execute_impl(NufftPlan* p, V* c, V* f) {
             p->partial_cap = pneed;
         }
     }
-    for (int b0 = 0; b0 < p->ntransf; b0 += p->max_batch) {
-        const int nb = std::min(p->max_batch, p->ntransf - b0);
+    // hook diagonals for the internal batch starting at transform b0
+    auto diag = [&](const void* ptr, int64_t sb, int type, int b0) {
+        DiagOp<V> d{};
+        if (ptr && type != HASTY_MULT_NONE) {
+            d.ptr = static_cast<const V*>(ptr) + int64_t(b0) * sb;
+            d.sb = sb;
+            d.sr = p->n[2];
+            d.type = type;
+        }
+        return d;
+    };
+    const V* pw = static_cast<const V*>(hk.pw);
+    for (int b0 = 0; b0 < ntransf; b0 += p->max_batch) {
+        const int nb = std::min(p->max_batch, ntransf - b0);
         V* cb = c + int64_t(b0) * p->M;
         V* fb = f + int64_t(b0) * nimg;
+        const DiagOp<V> m1 = diag(hk.m1, hk.m1_sb, hk.m1_type, b0);
+        const DiagOp<V> m2 = diag(hk.m2, hk.m2_sb, hk.m2_type, b0);
         GridArgs<V> ga{};
+        ga.pw = pw;
+        ga.accumulate = hk.accumulate;
         ga.fine = fine; ga.cdata = cb; ga.csort = static_cast<const V*>(p->csort);
         ga.sidx = p->sidx; ga.sl0 = p->sl0; ga.sw = static_cast<const R*>(p->sw);
         ga.visits = p->visits;
@@ -744,8 +760,7 @@ void execute_impl(NufftPlan* p, V* c, V* f) {
         ga.partial = static_cast<V*>(p->partial);
         const unsigned ngroups = unsigned(ceil_div(nb, ga.group));
         if (p->type == 2) {
-            DiagOp<V> none{};
-            pruned_forward<V>(pg, nb, fb, fine, bufA, bufB, p->sign, none, none, deapod, s);
+            pruned_forward<V>(pg, nb, fb, fine, bufA, bufB, p->sign, m1, m2, deapod, s);
             if (p->M == 0) continue;
             if (p->gm) {
                 HASTY_PROF("nufft_interp_gm", s);
@@ -778,7 +793,7 @@ void execute_impl(NufftPlan* p, V* c, V* f) {
                     HASTY_PROF("nufft_sort_values", s);
                     hipLaunchKernelGGL(sort_values_kernel<V>, dim3(unsigned(ceil_div(p->M, 64))), dim3(256),
                                        0, s, static_cast<const V*>(cb), static_cast<V*>(p->csort),
-                                       static_cast<const int*>(p->sinv), p->M, nb);
+                                       static_cast<const int*>(p->sinv), pw, p->M, nb);
                     HASTY_CHECK_HIP(hipGetLastError());
                 }
                 ga.subs = p->sub_spread;
@@ -805,9 +820,9 @@ void execute_impl(NufftPlan* p, V* c, V* f) {
                     HASTY_CHECK_HIP(hipGetLastError());
                 }
             }
-            DiagOp<V> none{};
-            pruned_inverse<V>(pg, nb, fine, fb, bufA, bufB, p->sign, R(1), none, none, none, deapod,
-                              0, 0, s);
+            const DiagOp<V> io = diag(hk.io, hk.io_sb, hk.io_type, b0);
+            pruned_inverse<V>(pg, nb, fine, hk.sum_batch ? f : fb, bufA, bufB, p->sign, R(1), m1, m2,
+                              io, deapod, hk.accumulate || (hk.sum_batch && b0 > 0), hk.sum_batch, s);
         }
     }
 }
@@ -878,8 +893,71 @@ void nufft_execute(NufftPlan* p, void* c, void* f) {
     if (!p->have_pts) throw Error(HASTY_ERR_STATE, "nufft execute called before setpts");
     HASTY_REQUIRE(f != nullptr, "mode array is NULL");
     HASTY_REQUIRE(c != nullptr || p->M == 0, "nonuniform array is NULL");
-    if (p->f64) execute_impl<double2>(p, static_cast<double2*>(c), static_cast<double2*>(f));
-    else execute_impl<float2>(p, static_cast<float2*>(c), static_cast<float2*>(f));
+    const NufftHooks none{};
+    if (p->f64) execute_impl<double2>(p, static_cast<double2*>(c), static_cast<double2*>(f), none, p->ntransf);
+    else execute_impl<float2>(p, static_cast<float2*>(c), static_cast<float2*>(f), none, p->ntransf);
+}
+
+void nufft_execute_hooked(NufftPlan* p, void* c, void* f, const NufftHooks& hk, int ntransf) {
+    if (!p->have_pts) throw Error(HASTY_ERR_STATE, "nufft execute called before setpts");
+    HASTY_REQUIRE(f != nullptr, "mode array is NULL");
+    HASTY_REQUIRE(c != nullptr || p->M == 0, "nonuniform array is NULL");
+    HASTY_REQUIRE(ntransf >= 1 && ntransf <= p->ntransf, "transform count out of range");
+    HASTY_REQUIRE(p->type == 1 || (hk.io == nullptr && !hk.sum_batch),
+                  "io diagonal and batch sum apply to type-1 plans");
+    if (p->f64) execute_impl<double2>(p, static_cast<double2*>(c), static_cast<double2*>(f), hk, ntransf);
+    else execute_impl<float2>(p, static_cast<float2*>(c), static_cast<float2*>(f), hk, ntransf);
+}
+
+void sense_offres_forward(NufftPlan* p, int ncoils, int L, const void* image, const void* smaps,
+                          const void* bil, const void* cjl, void* kdata) {
+    HASTY_REQUIRE(p->type == 2, "the SENSE forward runs on a type-2 plan");
+    HASTY_REQUIRE(ncoils >= 1 && L >= 1, "ncoils and L must be >= 1");
+    HASTY_REQUIRE(image && smaps && (kdata || p->M == 0), "image, smaps and kdata must be non-NULL");
+    HASTY_REQUIRE(L == 1 || (bil && cjl), "bil and cjl are required for L > 1");
+    const int64_t es = p->f64 ? 16 : 8;
+    const int64_t nimg = p->n[0] * p->n[1] * p->n[2];
+    auto at = [&](const void* base, int64_t off) {
+        return base ? static_cast<const char*>(base) + off * es : nullptr;
+    };
+    for (int c0 = 0; c0 < ncoils; c0 += p->ntransf) {
+        const int cb = std::min(p->ntransf, ncoils - c0);
+        for (int l = 0; l < L; ++l) {
+            NufftHooks hk{};
+            hk.m1 = image; hk.m1_sb = 0; hk.m1_type = HASTY_MULT;
+            hk.m2 = at(cjl, int64_t(l) * nimg); hk.m2_sb = 0; hk.m2_type = HASTY_MULT;
+            hk.pw = at(bil, int64_t(l) * p->M);
+            hk.accumulate = l > 0;
+            nufft_execute_hooked(p, const_cast<char*>(at(kdata, int64_t(c0) * p->M)),
+                                 const_cast<char*>(at(smaps, int64_t(c0) * nimg)), hk, cb);
+        }
+    }
+}
+
+void sense_offres_adjoint(NufftPlan* p, int ncoils, int L, const void* kdata, const void* smaps,
+                          const void* bil, const void* cjl, void* images, int combine) {
+    HASTY_REQUIRE(p->type == 1, "the SENSE adjoint runs on a type-1 plan");
+    HASTY_REQUIRE(ncoils >= 1 && L >= 1, "ncoils and L must be >= 1");
+    HASTY_REQUIRE(images && smaps && (kdata || p->M == 0), "kdata, smaps and images must be non-NULL");
+    HASTY_REQUIRE(L == 1 || (bil && cjl), "bil and cjl are required for L > 1");
+    const int64_t es = p->f64 ? 16 : 8;
+    const int64_t nimg = p->n[0] * p->n[1] * p->n[2];
+    auto at = [&](const void* base, int64_t off) {
+        return base ? static_cast<const char*>(base) + off * es : nullptr;
+    };
+    for (int c0 = 0; c0 < ncoils; c0 += p->ntransf) {
+        const int cb = std::min(p->ntransf, ncoils - c0);
+        for (int l = 0; l < L; ++l) {
+            NufftHooks hk{};
+            hk.m1 = at(cjl, int64_t(l) * nimg); hk.m1_sb = 0; hk.m1_type = HASTY_MULT_CONJ;
+            hk.m2 = at(smaps, int64_t(c0) * nimg); hk.m2_sb = nimg; hk.m2_type = HASTY_MULT_CONJ;
+            hk.pw = at(bil, int64_t(l) * p->M);
+            hk.accumulate = l > 0 || (combine && c0 > 0);
+            hk.sum_batch = combine;
+            void* out = combine ? images : static_cast<char*>(images) + int64_t(c0) * nimg * es;
+            nufft_execute_hooked(p, const_cast<char*>(at(kdata, int64_t(c0) * p->M)), out, hk, cb);
+        }
+    }
 }
 
 }  // namespace hasty

@@ -82,6 +82,36 @@ NufftPlan* nufft_makeplan(int type, int dim, const int64_t* nmodes, int sign, in
                           double tol, bool f64, const hasty_nufft_opts* opts, hipStream_t stream);
 void nufft_setpts(NufftPlan* p, int64_t M, const void* c0, const void* c1, const void* c2);
 void nufft_execute(NufftPlan* p, void* c, void* f);
+
+// Diagonal factors folded into one execution of a plan (the SENSE operators).
+// Arrays have the plan's complex type; image-shaped ones have row stride n_x
+// and batch stride `*_sb` (0 = the same array for every transform).
+//   type 2: the mode array read is  f_b (.) m1 (.) m2   (types MULT / MULT_CONJ)
+//           and  c_b[j] = pw[j] * interp_b(j)  (+ c_b[j] if accumulate);
+//   type 1: the spread input is  conj(pw[j]) * c_b[j]  and the stored image is
+//           f_b = result (.) m1 (.) m2 (.) io  (+ f_b if accumulate), summed
+//           over the transforms into one image when sum_batch.
+struct NufftHooks {
+    const void* m1 = nullptr; int64_t m1_sb = 0; int m1_type = HASTY_MULT_NONE;
+    const void* m2 = nullptr; int64_t m2_sb = 0; int m2_type = HASTY_MULT_NONE;
+    const void* io = nullptr; int64_t io_sb = 0; int io_type = HASTY_MULT_NONE;
+    const void* pw = nullptr;
+    int accumulate = 0;
+    int sum_batch = 0;
+};
+// Execute the first `ntransf` (<= plan ntransf) transforms with hooks.
+void nufft_execute_hooked(NufftPlan* p, void* c, void* f, const NufftHooks& hk, int ntransf);
+
+// Time-segmented SENSE (off-resonance) operators over a plan whose transforms
+// are a chunk of coils, L segments:
+//   forward: y_c = sum_l bil_l (.) type2(cjl_l (.) S_c (.) x)          [C, M]
+//   adjoint: x_c = conj(S_c) (.) sum_l conj(cjl_l) (.) type1(conj(bil_l) (.) y_c)
+//            [C, n...], or summed over coils into one image when `combine`.
+// bil [L, M] and cjl [L, n...] may be NULL (all ones).
+void sense_offres_forward(NufftPlan* p, int ncoils, int L, const void* image, const void* smaps,
+                          const void* bil, const void* cjl, void* kdata);
+void sense_offres_adjoint(NufftPlan* p, int ncoils, int L, const void* kdata, const void* smaps,
+                          const void* bil, const void* cjl, void* images, int combine);
 // out[a][j] = Re sum_t conj(w[t][j]) d/dk_a (type2(h)_t)_j on a type-2 plan
 void nufft_coord_grad(NufftPlan* p, const void* h, const void* w, void* out);
 

@@ -46,8 +46,20 @@ struct GridArgs {
     int nbatch, group, w, ws;
     V* partial;             // [slot][nbatch][tile cells]: partial tiles of split tiles
     int nsub, ngroups;      // interpolation: subproblems and transform groups (xcd_block)
+    const V* pw;            // optional per-point factor [M], caller order (SENSE time segments)
+    int accumulate;         // interpolation: add into cdata
     Geo g;
 };
+
+// Interpolation output of point `orig` of transform row `row`: pw[orig] * v,
+// plus the value already there when accumulating.
+template <class V>
+__device__ inline void store_point(const GridArgs<V>& a, int64_t row, int orig, V v) {
+    V* dst = a.cdata + row * a.M + orig;
+    if (a.pw) v = cmul(v, a.pw[orig]);
+    if (a.accumulate) v = cadd(v, *dst);
+    *dst = v;
+}
 
 // Interpolation blocks are numbered so that the transform groups of one
 // subproblem are consecutive in dispatch order on one XCD: workgroups go to
@@ -158,7 +170,7 @@ __global__ void __launch_bounds__(kNufftThreads) interp_kernel(GridArgs<V> a) {
                     acc.y += wz_[qz] * accy.y;
                 }
             }
-            a.cdata[int64_t(g0 + gi) * a.M + orig] = acc;
+            store_point(a, int64_t(g0 + gi), orig, acc);
         }
     }
 }
@@ -349,7 +361,7 @@ __global__ void __launch_bounds__(256) interp_gm_kernel(GridArgs<V> a) {
             acc.x += w[qz] * accy.x;
             acc.y += w[qz] * accy.y;
         }
-        a.cdata[int64_t(t) * a.M + orig] = acc;
+        store_point(a, int64_t(t), orig, acc);
     }
 }
 
@@ -373,7 +385,8 @@ __global__ void __launch_bounds__(256) spread_gm_kernel(GridArgs<V> a) {
     const int64_t G = a.g.nf[0] * a.g.nf[1] * a.g.nf[2];
     for (int t = 0; t < a.nbatch; ++t) {
         V* g = a.fine + int64_t(t) * G;
-        const V c = a.cdata[int64_t(t) * a.M + orig];
+        V c = a.cdata[int64_t(t) * a.M + orig];
+        if (a.pw) c = cmulc(c, a.pw[orig]);
         for (int qz = 0; qz < a.g.wa[0]; ++qz) {
             int z = l0z + qz; z -= z >= int(a.g.nf[0]) ? int(a.g.nf[0]) : 0;
             const R wzq = w[qz];
@@ -392,21 +405,27 @@ __global__ void __launch_bounds__(256) spread_gm_kernel(GridArgs<V> a) {
     }
 }
 
-// csort[sinv[j]][b] = c[b][j]: the nonuniform values in sorted point order,
-// transforms fastest.  A block reads 64 consecutive points of up to 32
+// csort[sinv[j]][b] = c[b][j] (times conj(pw[j]) when pw is given): the
+// nonuniform values in sorted point order, transforms fastest.  A block reads 64 consecutive points of up to 32
 // transforms (coalesced rows), transposes them in LDS and writes each point's
 // transforms as one contiguous run.
 template <class V>
 __global__ void __launch_bounds__(256) sort_values_kernel(const V* __restrict__ c, V* __restrict__ csort,
-                                                          const int* __restrict__ sinv, int64_t M, int B) {
+                                                          const int* __restrict__ sinv,
+                                                          const V* __restrict__ pw, int64_t M, int B) {
     __shared__ V tile[64][33];
     const int64_t j0 = int64_t(blockIdx.x) * 64;
     const int lane = threadIdx.x & 63, k0 = threadIdx.x >> 6;
     const int64_t j = j0 + lane;
     for (int b0 = 0; b0 < B; b0 += 32) {
         const int nbc = min(32, B - b0);
-        if (j < M)
-            for (int k = k0; k < nbc; k += 4) tile[lane][k] = c[int64_t(b0 + k) * M + j];
+        if (j < M) {
+            const V f = pw ? cconj(pw[j]) : cmake<V>(1, 0);
+            for (int k = k0; k < nbc; k += 4) {
+                const V v = c[int64_t(b0 + k) * M + j];
+                tile[lane][k] = pw ? cmul(v, f) : v;
+            }
+        }
         __syncthreads();
         for (int idx = threadIdx.x; idx < 64 * nbc; idx += 256) {
             const int pnt = idx / nbc, k = idx - pnt * nbc;

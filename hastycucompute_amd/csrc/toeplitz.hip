@@ -195,35 +195,170 @@ static void toeplitz_kernel_impl(int dim, const int64_t* n, int64_t M, const voi
 }
 
 // ---------------------------------------------------------------------------
-// SENSE normal operator
+// SENSE normal operators
 // ---------------------------------------------------------------------------
+// out[i][p] = sum_j Wm[i][j] in[j][p] with Wm = W ([C, C], row-major) or W^H
+// (herm): a C x C mix of stacked images, one pixel per thread, 16 outputs per
+// pass in registers.  W's index is wave-uniform, so its loads are scalar.
+__global__ void __launch_bounds__(256) stack_mix_kernel(const float2* __restrict__ W,
+                                                        const float2* __restrict__ in,
+                                                        float2* __restrict__ out, int C,
+                                                        int64_t nimg, int herm) {
+    const int64_t px = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (px >= nimg) return;
+    constexpr int U = 16;
+    for (int i0 = 0; i0 < C; i0 += U) {
+        float2 acc[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc[u] = make_float2(0.f, 0.f);
+        for (int j = 0; j < C; ++j) {
+            const float2 v = in[int64_t(j) * nimg + px];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (i0 + u < C) {
+                    const float2 w = herm ? cconj(W[int64_t(j) * C + i0 + u]) : W[int64_t(i0 + u) * C + j];
+                    acc[u].x += w.x * v.x - w.y * v.y;
+                    acc[u].y += w.x * v.y + w.y * v.x;
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (i0 + u < C) out[int64_t(i0 + u) * nimg + px] = acc[u];
+    }
+}
+
+static void stack_mix(const float2* W, const float2* in, float2* out, int C, int64_t nimg, int herm,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(stack_mix_kernel, dim3(unsigned(ceil_div(nimg, 256))), dim3(256), 0, s, W, in,
+                       out, C, nimg, herm);
+    HASTY_CHECK_HIP(hipGetLastError());
+}
+
+// One handle for the normal-operator family of the reference
+// (cpp/corelib/lib/mri/normal.cppm):
+//   kind 1 (NORMAL_T_T1_OP, NORMAL_IDT_T1_OP, NORMAL_IDTW_T1_OP), x [n] -> [n]:
+//     out = sum_c conj(S'_c) (.) sum_l conj(R_l) (.) T_l(R_l (.) S_c (.) x)
+//     with S' = S, or S'_c = sum_i conj(W_ic) S_i when weights are given: the
+//     reference's  sum_i conj(S_i) sum_j W_ij stack_j  regrouped as
+//     sum_j conj(sum_i conj(W_ij) S_i) stack_j, so the coil sum stays fused
+//     into the last FFT pass.  S' is formed once, at creation.
+//   kind 2 (NORMAL_T_T2_OP, NORMAL_IDT_T2_OP, NORMAL_IDTW_T2_OP), Psi [B, n] -> [B, n]:
+//     out_i = sum_j W_ij conj(x) (.) sum_l conj(R_l) (.) T_l(R_l (.) x (.) Psi_j)
+//     (the map Psi_j -> stack_j is the same linear map for every j, so the
+//     reference's mix of the outputs equals a mix of the inputs, done first).
+// R_l (kerneldiags) may be absent (all ones).
 struct NormalOp {
     ToeplitzDims d;
-    int ncoils = 0, coil_batch = 0;
-    const float2* smaps = nullptr;
-    const float2* kernel = nullptr;
+    int kind = 1;
+    int nstack = 0, chunk = 0;          // coils / stacked images, and how many per launch
+    int nkernels = 1;
+    const float2* stack = nullptr;      // kind 1: coil maps S [C, n]
+    const float2* out_maps = nullptr;   // kind 1: S' (== stack, or owned_maps)
+    const float2* x = nullptr;          // kind 2: the image X [n]
+    const float2* kernels = nullptr;    // [L, 2n...]
+    const float2* kerneldiags = nullptr;  // [L, n] or NULL
+    const float2* weights = nullptr;    // kind 2: [B, B] or NULL
+    float2* owned_maps = nullptr;
+    float2* mixbuf = nullptr;
     float2* bufA = nullptr;
     float2* bufB = nullptr;
     hipStream_t stream = nullptr;
     ~NormalOp() {
-        if (bufA) (void)hipFree(bufA);
-        if (bufB) (void)hipFree(bufB);
+        for (float2* p : {owned_maps, mixbuf, bufA, bufB})
+            if (p) (void)hipFree(p);
     }
 };
 
-static void normal_apply(NormalOp* op, const float2* x, float2* out) {
+static void normal_apply(NormalOp* op, const float2* in, float2* out) {
     const ToeplitzDims& d = op->d;
-    for (int c0 = 0; c0 < op->ncoils; c0 += op->coil_batch) {
-        const int cb = std::min(op->coil_batch, op->ncoils - c0);
-        const float2* S = op->smaps + int64_t(c0) * d.nimg();
-        DiagOp<float2> xin{};   // S_c (.) x   (x broadcast over coils)
-        xin.ptr = x; xin.sb = 0; xin.sr = d.n[2]; xin.type = HASTY_MULT;
-        DiagOp<float2> io{};    // (.) conj(S_c), summed over coils
-        io.ptr = S; io.sb = d.nimg(); io.sr = d.n[2]; io.type = HASTY_MULT_CONJ;
-        DiagOp<float2> none{};
-        toeplitz_core(d, cb, S, out, op->kernel, op->bufA, op->bufB, xin, none, none, none, io,
-                      c0 > 0 ? 1 : 0, 1, op->stream);
+    const int64_t nimg = d.nimg(), nK = d.nK();
+    auto bcast = [&](const float2* p, int type) {
+        DiagOp<float2> o{};
+        if (p) { o.ptr = p; o.sb = 0; o.sr = d.n[2]; o.type = type; }
+        return o;
+    };
+    const DiagOp<float2> none{};
+    if (op->kind == 1) {
+        for (int l = 0; l < op->nkernels; ++l) {
+            const float2* K = op->kernels + int64_t(l) * nK;
+            const float2* R = op->kerneldiags ? op->kerneldiags + int64_t(l) * nimg : nullptr;
+            for (int c0 = 0; c0 < op->nstack; c0 += op->chunk) {
+                const int cb = std::min(op->chunk, op->nstack - c0);
+                DiagOp<float2> io{};   // (.) conj(S'_c), summed over coils
+                io.ptr = op->out_maps + int64_t(c0) * nimg; io.sb = nimg; io.sr = d.n[2];
+                io.type = HASTY_MULT_CONJ;
+                toeplitz_core(d, cb, op->stack + int64_t(c0) * nimg, out, K, op->bufA, op->bufB,
+                              bcast(in, HASTY_MULT), bcast(R, HASTY_MULT), none,
+                              bcast(R, HASTY_MULT_CONJ), io, (l > 0 || c0 > 0) ? 1 : 0, 1, op->stream);
+            }
+        }
+    } else {
+        const float2* src = in;
+        if (op->weights) {
+            stack_mix(op->weights, in, op->mixbuf, op->nstack, nimg, 0, op->stream);
+            src = op->mixbuf;
+        }
+        for (int c0 = 0; c0 < op->nstack; c0 += op->chunk) {
+            const int cb = std::min(op->chunk, op->nstack - c0);
+            for (int l = 0; l < op->nkernels; ++l) {
+                const float2* K = op->kernels + int64_t(l) * nK;
+                const float2* R = op->kerneldiags ? op->kerneldiags + int64_t(l) * nimg : nullptr;
+                toeplitz_core(d, cb, src + int64_t(c0) * nimg, out + int64_t(c0) * nimg, K, op->bufA,
+                              op->bufB, bcast(op->x, HASTY_MULT), bcast(R, HASTY_MULT),
+                              bcast(op->x, HASTY_MULT_CONJ), bcast(R, HASTY_MULT_CONJ), none,
+                              l > 0 ? 1 : 0, 0, op->stream);
+            }
+        }
     }
+}
+
+static void* normal_alloc(size_t bytes) {
+    void* p = nullptr;
+    if (bytes && hipMalloc(&p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(HASTY_ERR_ALLOC, "normal operator workspace allocation failed");
+    }
+    return p;
+}
+
+static NormalOp* normal_make(int kind, int dim, const int64_t* n, int nstack, const void* stack,
+                             const void* x, int nkernels, const void* kernels,
+                             const void* kerneldiags, const void* weights, int chunk,
+                             hipStream_t stream) {
+    HASTY_REQUIRE(nstack >= 1, "the number of coils / stacked images must be >= 1");
+    HASTY_REQUIRE(nkernels >= 1, "the number of kernels must be >= 1");
+    HASTY_REQUIRE(kernels != nullptr, "kernels must be non-NULL");
+    HASTY_REQUIRE(kind == 2 || stack != nullptr, "smaps must be non-NULL");
+    HASTY_REQUIRE(kind == 1 || x != nullptr, "x must be non-NULL");
+    auto op = std::make_unique<NormalOp>();
+    op->d = toeplitz_dims(dim, n);
+    op->kind = kind;
+    op->nstack = nstack;
+    op->chunk = chunk > 0 ? std::min(chunk, nstack) : nstack;
+    op->nkernels = nkernels;
+    op->stack = static_cast<const float2*>(stack);
+    op->out_maps = op->stack;
+    op->x = static_cast<const float2*>(x);
+    op->kernels = static_cast<const float2*>(kernels);
+    op->kerneldiags = static_cast<const float2*>(kerneldiags);
+    op->stream = stream;
+    const int64_t nimg = op->d.nimg();
+    if (weights) {
+        const size_t bytes = size_t(nstack) * size_t(nimg) * 8;
+        if (kind == 1) {
+            op->owned_maps = static_cast<float2*>(normal_alloc(bytes));
+            stack_mix(static_cast<const float2*>(weights), op->stack, op->owned_maps, nstack, nimg, 1,
+                      stream);
+            op->out_maps = op->owned_maps;
+        } else {
+            op->weights = static_cast<const float2*>(weights);
+            op->mixbuf = static_cast<float2*>(normal_alloc(bytes));
+        }
+    }
+    op->bufA = static_cast<float2*>(normal_alloc(size_t(op->chunk) * op->d.nA() * 8));
+    op->bufB = static_cast<float2*>(normal_alloc(size_t(op->chunk) * op->d.nB() * 8));
+    return op.release();
 }
 
 }  // namespace hasty
@@ -312,22 +447,32 @@ int hasty_normal_create(int dim, const int64_t* n, int ncoils, const void* smaps
                         const void* kernel, int coil_batch, void* stream, hasty_normal_op* out) {
     return guarded([&] {
         HASTY_REQUIRE(out != nullptr, "output handle pointer is NULL");
-        HASTY_REQUIRE(ncoils >= 1, "ncoils must be >= 1");
         HASTY_REQUIRE(smaps && kernel, "smaps and kernel must be non-NULL");
-        auto op = std::make_unique<NormalOp>();
-        op->d = toeplitz_dims(dim, n);
-        op->ncoils = ncoils;
-        op->coil_batch = coil_batch > 0 ? std::min(coil_batch, ncoils) : ncoils;
-        op->smaps = static_cast<const float2*>(smaps);
-        op->kernel = static_cast<const float2*>(kernel);
-        op->stream = static_cast<hipStream_t>(stream);
-        const size_t a = size_t(op->coil_batch) * op->d.nA() * 8;
-        const size_t b = size_t(op->coil_batch) * op->d.nB() * 8;
-        if (hipMalloc(&op->bufA, a) != hipSuccess || hipMalloc(&op->bufB, b) != hipSuccess) {
-            (void)hipGetLastError();
-            throw Error(HASTY_ERR_ALLOC, "normal operator workspace allocation failed");
-        }
-        *out = reinterpret_cast<hasty_normal_op>(op.release());
+        *out = reinterpret_cast<hasty_normal_op>(normal_make(1, dim, n, ncoils, smaps, nullptr, 1, kernel,
+                                                             nullptr, nullptr, coil_batch,
+                                                             static_cast<hipStream_t>(stream)));
+    });
+}
+
+int hasty_normal_idt_create(int dim, const int64_t* n, int ncoils, const void* smaps, int nkernels,
+                            const void* kernels, const void* kerneldiags, const void* weights,
+                            int coil_batch, void* stream, hasty_normal_op* out) {
+    return guarded([&] {
+        HASTY_REQUIRE(out != nullptr, "output handle pointer is NULL");
+        *out = reinterpret_cast<hasty_normal_op>(normal_make(1, dim, n, ncoils, smaps, nullptr, nkernels,
+                                                             kernels, kerneldiags, weights, coil_batch,
+                                                             static_cast<hipStream_t>(stream)));
+    });
+}
+
+int hasty_normal_t2_create(int dim, const int64_t* n, int nstack, const void* x, int nkernels,
+                           const void* kernels, const void* kerneldiags, const void* weights,
+                           int stack_batch, void* stream, hasty_normal_op* out) {
+    return guarded([&] {
+        HASTY_REQUIRE(out != nullptr, "output handle pointer is NULL");
+        *out = reinterpret_cast<hasty_normal_op>(normal_make(2, dim, n, nstack, nullptr, x, nkernels,
+                                                             kernels, kerneldiags, weights, stack_batch,
+                                                             static_cast<hipStream_t>(stream)));
     });
 }
 
@@ -335,6 +480,7 @@ int hasty_normal_apply(hasty_normal_op h, const void* x, void* out) {
     return guarded([&] {
         HASTY_REQUIRE(h != nullptr, "normal operator handle is NULL");
         HASTY_REQUIRE(x && out, "x and out must be non-NULL");
+        // out is written by the first launch while x is still read by later ones
         HASTY_REQUIRE(x != out, "x and out must not alias");
         normal_apply(reinterpret_cast<NormalOp*>(h), static_cast<const float2*>(x),
                      static_cast<float2*>(out));

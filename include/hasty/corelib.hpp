@@ -10,6 +10,10 @@
 //   fft::transform_toeplitz_kernel, fft::toeplitz_multiplication
 //                                              cpp/corelib/lib/fft/toeplitz.cppm:47-93
 //   NORMAL_T_T1_OP<cuda_t,c64_t,DIM>          cpp/corelib/lib/mri/normal.cppm:43-93
+//   NORMAL_T_T2_OP, NORMAL_IDT_T1_OP, NORMAL_IDT_T2_OP,
+//   NORMAL_IDTW_T1_OP, NORMAL_IDTW_T2_OP       cpp/corelib/lib/mri/normal.cppm:113-735
+//   sense_forward_offresonance                 cpp/corelib/lib/mri/forward.cppm:36-165
+//   sense_adjoint_offresonance                 cpp/corelib/lib/mri/adjoint.cppm:33-171
 //   conjugate_gradient(sptr<Op>, max_iter, tol) cpp/corelib/lib/op/cg.cppm:122-184
 //   max_eig(A, x0, max_iter)                   cpp/corelib/lib/op/alg.cppm:13-30
 //
@@ -280,6 +284,8 @@ template <class TT, size_t DIM, nufft_type NT> struct nufft_plan<cuda_t, TT, DIM
 
     const std::array<int64_t, DIM>& nmodes() const { return _opts.nmodes; }
     int32_t ntransf() const { return _opts.ntransf; }
+    int64_t npoints() const { return _M; }
+    hasty_nufft_plan handle() const { return _plan; }
     TT tol() const { return _opts.tol; }
     nufft_opts<cuda_t, TT, DIM>& pa_opts() { return _opts; }
     const nufft_opts<cuda_t, TT, DIM>& pa_opts() const { return _opts; }
@@ -471,6 +477,272 @@ private:
     std::array<int64_t, DIM> _n{};
     std::shared_ptr<hasty_normal_op_s> _op;
 };
+
+// ---- multi-kernel / whitened normal operators (normal.cppm:113-735) -------------
+// kind 1 (image -> image, coil sum fused):
+//   out = sum_c conj(S'_c) . sum_l conj(R_l) . T_l(R_l . S_c . x)
+// kind 2 (stacked images -> stacked images):
+//   out_i = sum_j W_ij conj(X) . sum_l conj(R_l) . T_l(R_l . X . Psi_j)
+// R (kerneldiags) may be empty (all ones); W is empty for the unweighted ops.
+namespace detail {
+template <size_t DIM> class normal_family {
+protected:
+    using K_T = tensor<cuda_t, c64_t, DIM + 1>;
+    normal_family(K_T&& kernels, K_T&& kerneldiags)
+        : _kernels(std::move(kernels)), _kerneldiags(std::move(kerneldiags)) {
+        for (size_t a = 0; a < DIM; ++a) {
+            if (_kernels.shape_arr()[a + 1] % 2) throw std::runtime_error("kernels must have extents 2N");
+            _n[a] = _kernels.shape_arr()[a + 1] / 2;
+        }
+        if (_kerneldiags.data_ptr())
+            for (size_t a = 0; a < DIM; ++a)
+                if (_kerneldiags.shape_arr()[a + 1] != _n[a])
+                    throw std::runtime_error("kerneldiags must have shape [L, N...]");
+    }
+    // the reference checks the counts when the operator is applied (normal.cppm:222-224)
+    void check_counts() const {
+        if (_kerneldiags.data_ptr() && _kerneldiags.template shape<0>() != _kernels.template shape<0>())
+            throw std::runtime_error("Number of kernels and kernel diagonals must match.");
+    }
+    int nkernels() const { return int(_kernels.template shape<0>()); }
+    void adopt(hasty_normal_op h) {
+        _op = std::shared_ptr<hasty_normal_op_s>(h, [](hasty_normal_op q) { (void)hasty_normal_destroy(q); });
+    }
+    K_T _kernels, _kerneldiags;
+    std::array<int64_t, DIM> _n{};
+    std::shared_ptr<hasty_normal_op_s> _op;
+};
+}  // namespace detail
+
+template <class D, class TT, size_t DIM> class NORMAL_IDT_T1_OP;
+template <class D, class TT, size_t DIM> class NORMAL_IDTW_T1_OP;
+template <class D, class TT, size_t DIM> class NORMAL_T_T2_OP;
+template <class D, class TT, size_t DIM> class NORMAL_IDT_T2_OP;
+template <class D, class TT, size_t DIM> class NORMAL_IDTW_T2_OP;
+
+namespace detail {
+template <size_t DIM> class normal_t1 : public normal_family<DIM> {
+public:
+    using device_type_t = cuda_t;
+    using input_tensor_type_t = c64_t;
+    using output_tensor_type_t = c64_t;
+    static constexpr size_t input_rank_t() { return DIM; }
+    static constexpr size_t output_rank_t() { return DIM; }
+
+    tensor<cuda_t, c64_t, DIM> operator()(const tensor<cuda_t, c64_t, DIM>& x) const {
+        auto out = tensor<cuda_t, c64_t, DIM>::empty(this->_n);
+        apply(x, out);
+        return out;
+    }
+    void apply(const tensor<cuda_t, c64_t, DIM>& x, tensor<cuda_t, c64_t, DIM>& out) const {
+        this->check_counts();
+        if (x.shape_arr() != this->_n || out.shape_arr() != this->_n) throw std::runtime_error("x must have shape N");
+        detail::check(hasty_normal_apply(this->_op.get(), x.data_ptr(), out.mutable_data_ptr()));
+    }
+
+protected:
+    normal_t1(tensor<cuda_t, c64_t, DIM + 1>&& kernels, tensor<cuda_t, c64_t, DIM + 1>&& kerneldiags,
+              tensor<cuda_t, c64_t, DIM + 1>&& stacked_diags, tensor<cuda_t, c64_t, 2>&& weights, int coil_batch)
+        : normal_family<DIM>(std::move(kernels), std::move(kerneldiags)), _smaps(std::move(stacked_diags)),
+          _weights(std::move(weights)) {
+        const int64_t C = _smaps.template shape<0>();
+        for (size_t a = 0; a < DIM; ++a)
+            if (_smaps.shape_arr()[a + 1] != this->_n[a]) throw std::runtime_error("stacked_diags must be [C, N...]");
+        if (_weights.data_ptr() && (_weights.template shape<0>() != C || _weights.template shape<1>() != C))
+            throw std::runtime_error("weights must be [C, C]");
+        if (this->_kerneldiags.data_ptr() && this->_kerneldiags.template shape<0>() != this->nkernels()) return;
+        hasty_normal_op h = nullptr;
+        detail::check(hasty_normal_idt_create(int(DIM), this->_n.data(), int(C), _smaps.data_ptr(), this->nkernels(),
+                                              this->_kernels.data_ptr(), this->_kerneldiags.data_ptr(),
+                                              _weights.data_ptr(), coil_batch, nullptr, &h));
+        this->adopt(h);
+    }
+    tensor<cuda_t, c64_t, DIM + 1> _smaps;
+    tensor<cuda_t, c64_t, 2> _weights;
+};
+
+template <size_t DIM> class normal_t2 : public normal_family<DIM> {
+public:
+    using device_type_t = cuda_t;
+    using input_tensor_type_t = c64_t;
+    using output_tensor_type_t = c64_t;
+    static constexpr size_t input_rank_t() { return DIM + 1; }
+    static constexpr size_t output_rank_t() { return DIM + 1; }
+
+    tensor<cuda_t, c64_t, DIM + 1> operator()(const tensor<cuda_t, c64_t, DIM + 1>& psi) {
+        auto out = tensor<cuda_t, c64_t, DIM + 1>::empty(psi.shape_arr());
+        apply(psi, out);
+        return out;
+    }
+    void apply(const tensor<cuda_t, c64_t, DIM + 1>& psi, tensor<cuda_t, c64_t, DIM + 1>& out) {
+        this->check_counts();
+        for (size_t a = 0; a < DIM; ++a)
+            if (psi.shape_arr()[a + 1] != this->_n[a]) throw std::runtime_error("stacked diagonals must be [B, N...]");
+        if (out.shape_arr() != psi.shape_arr()) throw std::runtime_error("out must match the stacked diagonals");
+        const int64_t B = psi.template shape<0>();
+        if (_weights.data_ptr() && _weights.template shape<0>() != B)
+            throw std::runtime_error("the number of stacked diagonals must match the weights");
+        if (B != _B) {
+            hasty_normal_op h = nullptr;
+            detail::check(hasty_normal_t2_create(int(DIM), this->_n.data(), int(B), _x.data_ptr(), this->nkernels(),
+                                                 this->_kernels.data_ptr(), this->_kerneldiags.data_ptr(),
+                                                 _weights.data_ptr(), _stack_batch, nullptr, &h));
+            this->adopt(h);
+            _B = B;
+        }
+        detail::check(hasty_normal_apply(this->_op.get(), psi.data_ptr(), out.mutable_data_ptr()));
+    }
+
+protected:
+    normal_t2(tensor<cuda_t, c64_t, DIM + 1>&& kernels, tensor<cuda_t, c64_t, DIM + 1>&& kerneldiags,
+              tensor<cuda_t, c64_t, DIM>&& x, tensor<cuda_t, c64_t, 2>&& weights, int stack_batch)
+        : normal_family<DIM>(std::move(kernels), std::move(kerneldiags)), _x(std::move(x)),
+          _weights(std::move(weights)), _stack_batch(stack_batch) {
+        if (_x.shape_arr() != this->_n) throw std::runtime_error("x must have shape N");
+        if (_weights.data_ptr() && _weights.template shape<0>() != _weights.template shape<1>())
+            throw std::runtime_error("weights must be square");
+    }
+    tensor<cuda_t, c64_t, DIM> _x;
+    tensor<cuda_t, c64_t, 2> _weights;
+    int _stack_batch = 0;
+    int64_t _B = -1;
+};
+}  // namespace detail
+
+// normal.cppm:185-272
+template <size_t DIM> class NORMAL_IDT_T1_OP<cuda_t, c64_t, DIM> : public detail::normal_t1<DIM> {
+public:
+    static constexpr const char* class_name = "NORMAL_IDT_T1_OP";
+    NORMAL_IDT_T1_OP(tensor<cuda_t, c64_t, DIM + 1>&& kernels, tensor<cuda_t, c64_t, DIM + 1>&& kerneldiags,
+                     tensor<cuda_t, c64_t, DIM + 1>&& stacked_diags, bool store_module = false, int coil_batch = 0)
+        : detail::normal_t1<DIM>(std::move(kernels), std::move(kerneldiags), std::move(stacked_diags),
+                                 tensor<cuda_t, c64_t, 2>(), coil_batch) {
+        (void)store_module;
+    }
+};
+
+// normal.cppm:405-564; the output-side maps S'_c = sum_i conj(W_ic) S_i are formed once
+template <size_t DIM> class NORMAL_IDTW_T1_OP<cuda_t, c64_t, DIM> : public detail::normal_t1<DIM> {
+public:
+    static constexpr const char* class_name = "NORMAL_IDTW_T1_OP";
+    NORMAL_IDTW_T1_OP(tensor<cuda_t, c64_t, DIM + 1>&& kernels, tensor<cuda_t, c64_t, DIM + 1>&& kerneldiags,
+                      tensor<cuda_t, c64_t, DIM + 1>&& stacked_diags, tensor<cuda_t, c64_t, 2>&& weights,
+                      bool store_module = false, int coil_batch = 0)
+        : detail::normal_t1<DIM>(std::move(kernels), std::move(kerneldiags), std::move(stacked_diags),
+                                 std::move(weights), coil_batch) {
+        (void)store_module;
+    }
+};
+
+// normal.cppm:113-160
+template <size_t DIM> class NORMAL_T_T2_OP<cuda_t, c64_t, DIM> : public detail::normal_t2<DIM> {
+public:
+    static constexpr const char* class_name = "NORMAL_T_T2_OP";
+    NORMAL_T_T2_OP(tensor<cuda_t, c64_t, DIM>&& kernel, tensor<cuda_t, c64_t, DIM>&& x, int stack_batch = 0)
+        : detail::normal_t2<DIM>(as_stack(std::move(kernel)), tensor<cuda_t, c64_t, DIM + 1>(), std::move(x),
+                                 tensor<cuda_t, c64_t, 2>(), stack_batch) {}
+
+private:
+    static tensor<cuda_t, c64_t, DIM + 1> as_stack(tensor<cuda_t, c64_t, DIM>&& k) {
+        std::array<int64_t, DIM + 1> s{};
+        s[0] = 1;
+        for (size_t a = 0; a < DIM; ++a) s[a + 1] = k.shape_arr()[a];
+        auto t = tensor<cuda_t, c64_t, DIM + 1>::empty(s);
+        detail::check_hip(hipMemcpy(t.mutable_data_ptr(), k.data_ptr(), size_t(k.numel()) * sizeof(c64_t),
+                                    hipMemcpyDeviceToDevice), "hipMemcpy D2D");
+        return t;
+    }
+};
+
+// normal.cppm:292-375
+template <size_t DIM> class NORMAL_IDT_T2_OP<cuda_t, c64_t, DIM> : public detail::normal_t2<DIM> {
+public:
+    static constexpr const char* class_name = "NORMAL_IDT_T2_OP";
+    NORMAL_IDT_T2_OP(tensor<cuda_t, c64_t, DIM + 1>&& kernels, tensor<cuda_t, c64_t, DIM + 1>&& kerneldiags,
+                     tensor<cuda_t, c64_t, DIM>&& x, int stack_batch = 0)
+        : detail::normal_t2<DIM>(std::move(kernels), std::move(kerneldiags), std::move(x),
+                                 tensor<cuda_t, c64_t, 2>(), stack_batch) {}
+};
+
+// normal.cppm:586-735
+template <size_t DIM> class NORMAL_IDTW_T2_OP<cuda_t, c64_t, DIM> : public detail::normal_t2<DIM> {
+public:
+    static constexpr const char* class_name = "NORMAL_IDTW_T2_OP";
+    NORMAL_IDTW_T2_OP(tensor<cuda_t, c64_t, DIM + 1>&& kernels, tensor<cuda_t, c64_t, DIM + 1>&& kerneldiags,
+                      tensor<cuda_t, c64_t, DIM>&& x, tensor<cuda_t, c64_t, 2>&& weights, bool store_module = false,
+                      int stack_batch = 0)
+        : detail::normal_t2<DIM>(std::move(kernels), std::move(kerneldiags), std::move(x), std::move(weights),
+                                 stack_batch) {
+        (void)store_module;
+    }
+};
+
+// ---- time-segmented SENSE (forward.cppm:36-165, adjoint.cppm:33-171) -----------
+// All coils of a chunk go through one plan of ntransf = coil_batch (0 = all);
+// tolerance as in the reference (1e-6 single, 1e-14 double precision).  The
+// reference takes host-cached tensors and a thread pool and returns host
+// tensors; here everything stays on the device.
+namespace detail {
+template <class TTC, size_t DIM, nufft_type NT>
+uptr<nufft_plan<cuda_t, TTC, DIM, NT>> sense_plan(const std::array<tensor<cuda_t, TTC, 1>, DIM>& coords,
+                                                   const std::array<int64_t, DIM>& n, int64_t C, int coil_batch) {
+    nufft_opts<cuda_t, TTC, DIM> o;
+    o.nmodes = n;
+    o.ntransf = int32_t(coil_batch > 0 ? std::min<int64_t>(coil_batch, C) : C);
+    o.tol = std::is_same_v<TTC, f32_t> ? TTC(1e-6) : TTC(1e-14);
+    o.sign = NT == nufft_type::TYPE_1 ? nufft_sign::DEFAULT_TYPE_1 : nufft_sign::DEFAULT_TYPE_2;
+    auto plan = nufft_plan<cuda_t, TTC, DIM, NT>::make(o);
+    plan->setpts(coords);
+    return plan;
+}
+template <class TT, size_t DIM>
+void check_offres(const tensor<cuda_t, TT, DIM + 1>& smaps, const tensor<cuda_t, TT, 2>& bil,
+                  const tensor<cuda_t, TT, DIM + 1>& cjl, int64_t M) {
+    if (bil.template shape<0>() != cjl.template shape<0>()) throw std::runtime_error("bil and cjl must have L rows");
+    if (bil.template shape<1>() != M) throw std::runtime_error("bil must be [L, M]");
+    for (size_t a = 0; a < DIM; ++a)
+        if (cjl.shape_arr()[a + 1] != smaps.shape_arr()[a + 1]) throw std::runtime_error("cjl must be [L, N...]");
+}
+}  // namespace detail
+
+template <class TT, class TTC, size_t DIM>
+tensor<cuda_t, TT, 2> sense_forward_offresonance(const std::array<tensor<cuda_t, TTC, 1>, DIM>& coords,
+                                                 const tensor<cuda_t, TT, DIM>& image,
+                                                 const tensor<cuda_t, TT, DIM + 1>& smaps,
+                                                 const tensor<cuda_t, TT, 2>& bil,
+                                                 const tensor<cuda_t, TT, DIM + 1>& cjl, int coil_batch = 0) {
+    static_assert(std::is_same_v<TT, complex_t<TTC>>, "data must be the plan's complex type");
+    const int64_t C = smaps.template shape<0>(), M = coords[0].template shape<0>();
+    std::array<int64_t, DIM> n{};
+    for (size_t a = 0; a < DIM; ++a) n[a] = smaps.shape_arr()[a + 1];
+    if (image.shape_arr() != n) throw std::runtime_error("image must have the maps' spatial shape");
+    detail::check_offres<TT, DIM>(smaps, bil, cjl, M);
+    auto plan = detail::sense_plan<TTC, DIM, nufft_type::TYPE_2>(coords, n, C, coil_batch);
+    auto out = tensor<cuda_t, TT, 2>::empty({C, M});
+    detail::check(hasty_sense_forward(plan->handle(), int(C), int(bil.template shape<0>()), image.data_ptr(),
+                                      smaps.data_ptr(), bil.data_ptr(), cjl.data_ptr(), out.mutable_data_ptr()));
+    return out;
+}
+
+template <class TT, class TTC, size_t DIM>
+tensor<cuda_t, TT, DIM + 1> sense_adjoint_offresonance(const std::array<tensor<cuda_t, TTC, 1>, DIM>& coords,
+                                                       const tensor<cuda_t, TT, 2>& kdata,
+                                                       const tensor<cuda_t, TT, DIM + 1>& smaps,
+                                                       const tensor<cuda_t, TT, 2>& bil,
+                                                       const tensor<cuda_t, TT, DIM + 1>& cjl, int coil_batch = 0) {
+    static_assert(std::is_same_v<TT, complex_t<TTC>>, "data must be the plan's complex type");
+    const int64_t C = smaps.template shape<0>(), M = coords[0].template shape<0>();
+    std::array<int64_t, DIM> n{};
+    for (size_t a = 0; a < DIM; ++a) n[a] = smaps.shape_arr()[a + 1];
+    if (kdata.template shape<0>() != C || kdata.template shape<1>() != M)
+        throw std::runtime_error("kdata must be [C, M]");
+    detail::check_offres<TT, DIM>(smaps, bil, cjl, M);
+    auto plan = detail::sense_plan<TTC, DIM, nufft_type::TYPE_1>(coords, n, C, coil_batch);
+    auto out = tensor<cuda_t, TT, DIM + 1>::empty(smaps.shape_arr());
+    detail::check(hasty_sense_adjoint(plan->handle(), int(C), int(bil.template shape<0>()), kdata.data_ptr(),
+                                      smaps.data_ptr(), bil.data_ptr(), cjl.data_ptr(), out.mutable_data_ptr(), 0));
+    return out;
+}
 
 // ---- CG (cg.cppm:122-184) and power iteration (alg.cppm:13-30) -----------------
 // Returns cg(x, b) -> x.  The step arithmetic runs in the library's FP64

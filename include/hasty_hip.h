@@ -141,9 +141,51 @@ int64_t hasty_toeplitz_scratch_bytes(int dim, const int64_t* n, int64_t batch);
  *      coil sum fused into the last pass (no [C, n...] intermediate). ------ */
 int hasty_normal_create(int dim, const int64_t* n, int ncoils, const void* smaps,
                         const void* kernel, int coil_batch, void* stream, hasty_normal_op* op);
+/* ---- multi-kernel / whitened normal operators, kind 1 (image -> image):
+ *      replaces NORMAL_IDT_T1_OP (normal.cppm:185-272) and NORMAL_IDTW_T1_OP
+ *      (normal.cppm:405-564):
+ *        out = sum_c conj(S'_c) . sum_l conj(R_l) . T_l(R_l . S_c . x)
+ *      kernels [L, 2n...], kerneldiags R [L, n...] (NULL = all ones),
+ *      weights W [C, C] (NULL = identity; else S'_c = sum_i conj(W_ic) S_i,
+ *      formed once here, which equals the reference's sum_i conj(S_i) sum_j W_ij stack_j).
+ *      smaps, kernels, kerneldiags and weights are caller memory, read by the
+ *      applies (weights and smaps also here); all c64.  Apply with
+ *      hasty_normal_apply(op, x [n...], out [n...]). */
+int hasty_normal_idt_create(int dim, const int64_t* n, int ncoils, const void* smaps, int nkernels,
+                            const void* kernels, const void* kerneldiags, const void* weights,
+                            int coil_batch, void* stream, hasty_normal_op* op);
+/* ---- kind 2 (stacked images -> stacked images): replaces NORMAL_T_T2_OP
+ *      (normal.cppm:113-160), NORMAL_IDT_T2_OP (normal.cppm:292-375) and
+ *      NORMAL_IDTW_T2_OP (normal.cppm:586-735):
+ *        out_i = sum_j W_ij conj(X) . sum_l conj(R_l) . T_l(R_l . X . Psi_j)
+ *      x = X [n...]; weights [B, B] or NULL; apply with
+ *      hasty_normal_apply(op, Psi [B, n...], out [B, n...]); stack_batch
+ *      caps the stacked images per launch (0 = all). */
+int hasty_normal_t2_create(int dim, const int64_t* n, int nstack, const void* x, int nkernels,
+                           const void* kernels, const void* kerneldiags, const void* weights,
+                           int stack_batch, void* stream, hasty_normal_op* op);
+/* x and out must be distinct buffers. */
 int hasty_normal_apply(hasty_normal_op op, const void* x, void* out);
 int hasty_normal_set_stream(hasty_normal_op op, void* stream);
 int hasty_normal_destroy(hasty_normal_op op);
+
+/* ---- time-segmented (off-resonance) SENSE over a NUFFT plan whose
+ *      transforms are a chunk of coils (coils run in chunks of the plan's
+ *      ntransf; data in the plan's precision). bil [L, M] and cjl [L, n...]
+ *      may both be NULL when L == 1 (plain SENSE).
+ * Replaces sense_forward_offresonance (forward.cppm:36-165), on a type-2 plan:
+ *   kdata[c] = sum_l bil[l] . type2(cjl[l] . smaps[c] . image)      [C, M]
+ * with the products folded into the plan's load and interpolation passes. */
+int hasty_sense_forward(hasty_nufft_plan plan, int ncoils, int L, const void* image,
+                        const void* smaps, const void* bil, const void* cjl, void* kdata);
+/* Replaces sense_adjoint_offresonance (adjoint.cppm:33-171), on a type-1 plan:
+ *   images[c] = conj(smaps[c]) . sum_l conj(cjl[l]) . type1(conj(bil[l]) . kdata[c])
+ * [C, n...]; with combine != 0 the coils are summed into one image [n...]
+ * inside the last FFT pass.  (The reference applies conj(smaps) twice when
+ * the NUFFT and data precisions match, adjoint.cppm:116-117; once here.) */
+int hasty_sense_adjoint(hasty_nufft_plan plan, int ncoils, int L, const void* kdata,
+                        const void* smaps, const void* bil, const void* cjl, void* images,
+                        int combine);
 
 /* ---- CG / vector ops (cg.cppm:122-184, alg.cppm:13-30) with FP64
  *      accumulation.  state = caller device buffer of HASTY_CG_STATE_DOUBLES

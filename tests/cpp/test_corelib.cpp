@@ -211,6 +211,78 @@ int main() {
         report("toeplitz_multiply vs NDFT", rel_l2(t.to_host(), back), 2e-5);
     }
 
+    // multi-kernel operators: kernels {T, T}, diagonals {1, R} with a spatial phase R
+    std::vector<c64_t> R(static_cast<size_t>(I));
+    for (int64_t i = 0; i < I; ++i) R[size_t(i)] = std::polar(1.f, 0.37f * float(i % 17) - 0.11f * float(i / 160));
+    auto idt_ref = ref_n;
+    for (int c = 0; c < p.C; ++c) {
+        std::vector<c64_t> f(static_cast<size_t>(I));
+        for (int64_t i = 0; i < I; ++i) f[size_t(i)] = R[size_t(i)] * Sx[size_t(c * I + i)];
+        const auto back = ndft1(p, ndft2(p, f, -1), +1);
+        for (int64_t i = 0; i < I; ++i) {
+            const std::complex<double> s(p.S[size_t(c * I + i)].real(), p.S[size_t(c * I + i)].imag());
+            const std::complex<double> r(R[size_t(i)].real(), R[size_t(i)].imag());
+            idt_ref[size_t(i)] += std::conj(s) * std::conj(r) * back[size_t(i)] / double(I);
+        }
+    }
+    const int64_t nK = kernel.numel();
+    std::vector<c64_t> hkern = kernel.to_host(), kk(size_t(2 * nK)), dd(size_t(2 * I));
+    std::copy(hkern.begin(), hkern.end(), kk.begin());
+    std::copy(hkern.begin(), hkern.end(), kk.begin() + nK);
+    std::fill(dd.begin(), dd.begin() + I, c64_t(1.f, 0.f));
+    std::copy(R.begin(), R.end(), dd.begin() + I);
+    const std::array<int64_t, 4> kshape{2, 2 * p.n[0], 2 * p.n[1], 2 * p.n[2]}, dshape{2, p.n[0], p.n[1], p.n[2]};
+    const std::array<int64_t, 4> sshape{p.C, p.n[0], p.n[1], p.n[2]};
+    {
+        NORMAL_IDT_T1_OP<cuda_t, c64_t, 3> idt(tensor<cuda_t, c64_t, 4>::from_host(kk, kshape),
+                                               tensor<cuda_t, c64_t, 4>::from_host(dd, dshape),
+                                               tensor<cuda_t, c64_t, 4>::from_host(p.S, sshape));
+        report("NORMAL_IDT_T1_OP (L=2) vs NDFT", rel_l2(idt(dx).to_host(), idt_ref), 2e-5);
+        std::vector<c64_t> W(size_t(p.C * p.C), c64_t(0.f, 0.f));
+        for (int c = 0; c < p.C; ++c) W[size_t(c * p.C + c)] = c64_t(2.f, 0.f);
+        NORMAL_IDTW_T1_OP<cuda_t, c64_t, 3> idtw(tensor<cuda_t, c64_t, 4>::from_host(kk, kshape),
+                                                 tensor<cuda_t, c64_t, 4>::from_host(dd, dshape),
+                                                 tensor<cuda_t, c64_t, 4>::from_host(p.S, sshape),
+                                                 tensor<cuda_t, c64_t, 2>::from_host(W, {p.C, p.C}));
+        auto twice = idt_ref;
+        for (auto& v : twice) v *= 2.0;
+        report("NORMAL_IDTW_T1_OP (W = 2 I) vs NDFT", rel_l2(idtw(dx).to_host(), twice), 2e-5);
+        // <Psi, T2(Psi)> with Psi = S equals <x, sum_c conj(S_c) T(S_c x)>
+        NORMAL_T_T2_OP<cuda_t, c64_t, 3> t2(kernel.clone(), dx.clone());
+        auto psi = tensor<cuda_t, c64_t, 4>::from_host(p.S, sshape);
+        const auto ht2 = t2(psi).to_host();
+        std::complex<double> lhs = 0, rhs = 0;
+        for (int64_t i = 0; i < p.C * I; ++i)
+            lhs += std::conj(std::complex<double>(p.S[size_t(i)])) * std::complex<double>(ht2[size_t(i)]);
+        for (int64_t i = 0; i < I; ++i) rhs += std::conj(std::complex<double>(p.x[size_t(i)])) * ref_n[size_t(i)];
+        report("NORMAL_T_T2_OP: <S, T2(S)> vs <x, N x>", std::abs(lhs - rhs) / std::abs(rhs), 2e-5);
+        expect_throw("kernel / diagonal count mismatch", "Number of kernels and kernel diagonals must match.", [&] {
+            NORMAL_IDT_T1_OP<cuda_t, c64_t, 3> bad(tensor<cuda_t, c64_t, 4>::from_host(kk, kshape),
+                                                   tensor<cuda_t, c64_t, 4>::from_host(R, {1, p.n[0], p.n[1], p.n[2]}),
+                                                   tensor<cuda_t, c64_t, 4>::from_host(p.S, sshape));
+            (void)bad(dx);
+        });
+    }
+
+    // time-segmented SENSE with one segment of unit factors is plain SENSE
+    {
+        auto bil = tensor<cuda_t, c64_t, 2>::from_host(std::vector<c64_t>(size_t(p.M), c64_t(1.f, 0.f)), {1, p.M});
+        auto cjl = tensor<cuda_t, c64_t, 4>::from_host(std::vector<c64_t>(size_t(I), c64_t(1.f, 0.f)),
+                                                       {1, p.n[0], p.n[1], p.n[2]});
+        auto maps = tensor<cuda_t, c64_t, 4>::from_host(p.S, sshape);
+        const auto y = sense_forward_offresonance<c64_t, f32_t, 3>(coords, dx, maps, bil, cjl, 2).to_host();
+        std::vector<c64_t> y1(y.begin() + p.M, y.begin() + 2 * p.M);
+        report("sense_forward_offresonance coil 1 vs NDFT", rel_l2(y1, ref2[1]), 2e-5);
+        const auto xa = sense_adjoint_offresonance<c64_t, f32_t, 3>(coords, dk, maps, bil, cjl).to_host();
+        std::vector<c64_t> k0(hk.begin(), hk.begin() + p.M);
+        std::vector<std::complex<double>> k0d(k0.begin(), k0.end());
+        auto back = ndft1(p, k0d, +1);
+        for (int64_t i = 0; i < I; ++i)
+            back[size_t(i)] *= std::conj(std::complex<double>(p.S[size_t(i)]));
+        std::vector<c64_t> x0(xa.begin(), xa.begin() + I);
+        report("sense_adjoint_offresonance coil 0 vs NDFT", rel_l2(x0, back), 2e-5);
+    }
+
     // CG on the normal equations: 30 iterations reduce the residual by 1e3
     {
         auto b = (*op)(dx);
