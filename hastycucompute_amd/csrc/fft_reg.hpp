@@ -175,6 +175,11 @@ template <class V> struct PassArgs {
     int accumulate, sum_batch;
     const V* tw;                        // tw[k] = exp(-2 pi i k / L)
     int ldp;                            // LDS stride per line (complex elements)
+    // byte strides as 32-bit values: within one (batch, o) slab every offset
+    // a thread forms fits in 32 bits (checked on the host), so addresses are
+    // a wave-uniform 64-bit base plus a 32-bit per-lane offset
+    uint32_t in_ib, in_sb, out_ib, out_sb, psf_ib, psf_sb;
+    uint32_t lm1_rb, lm2_rb, sm1_rb, sm2_rb, sio_rb;   // diagonal row strides in bytes
 };
 
 template <int L1, int L2> struct RegShape {
@@ -182,6 +187,7 @@ template <int L1, int L2> struct RegShape {
     static constexpr int T = L1 > L2 ? L1 : L2;
 };
 
+// v * w (SIGN < 0) or v * conj(w) (SIGN > 0) with w read from LDS
 template <class V, int SIGN>
 __device__ inline V twiddle_mul(V v, const V* tw, int idx) {
     V w = tw[idx];
@@ -197,13 +203,78 @@ enum PassMode : int {
     PM_ROW_STORE = 3,  // contiguous lines, diagonals / deapodisation / sums on store
 };
 
-template <class V, int L1, int L2, int SIGN, int MODE>
+// Line shapes known at compile time (the Toeplitz passes): the input is the
+// first half of the line (in_shift 0, in_len L/2), or the output keeps
+// positions [L/2 - 1, L - 1) (out_shift L/2 - 1, out_len L/2).  Positions
+// that are zero or dropped then cost nothing: no load, no arithmetic on the
+// zeros (the passes are built with -fno-signed-zeros so x + 0 folds), no
+// DFT output that is not stored.
+enum PassSpec : int { PS_NONE = 0, PS_HALF_IN = 1, PS_HALF_OUT = 2 };
+
+// a * b (sgn = 1) or a * conj(b) (sgn = -1); sgn is wave-uniform
+template <class V>
+__device__ inline V cmul_sgn(V a, V b, decltype(V::x) sgn) {
+    b.y *= sgn;
+    return cmul(a, b);
+}
+
+// NT: streaming access (nontemporal hint) for the line data of the column
+// passes, which is read or written exactly once per pass
+template <class V, bool NT = false>
+__device__ inline V load_at(const char* base, uint32_t off) {
+    if constexpr (NT) {
+        const decltype(V::x)* p = reinterpret_cast<const decltype(V::x)*>(base + off);
+        V v;
+        v.x = __builtin_nontemporal_load(p);
+        v.y = __builtin_nontemporal_load(p + 1);
+        return v;
+    } else {
+        return *reinterpret_cast<const V*>(base + off);
+    }
+}
+template <class V, bool NT = false>
+__device__ inline void store_at(char* base, uint32_t off, V v) {
+    if constexpr (NT) {
+        decltype(V::x)* p = reinterpret_cast<decltype(V::x)*>(base + off);
+        __builtin_nontemporal_store(v.x, p);
+        __builtin_nontemporal_store(v.y, p + 1);
+    } else {
+        *reinterpret_cast<V*>(base + off) = v;
+    }
+}
+
+// v[m] (.)= d[row, j[m]] for the positions with ok[m]: all loads of the
+// diagonal are issued before the first product
+template <class V, int NM>
+__device__ inline void apply_diag(V* v, const DiagOp<V>& d, int64_t b, uint32_t row_off, const int* j,
+                                  const bool* ok) {
+    if (!d.ptr) return;
+    const char* base = reinterpret_cast<const char*>(d.ptr + b * d.sb);
+    const decltype(V::x) sgn = d.type == HASTY_MULT_CONJ ? -1 : 1;
+    V w[NM];
+#pragma unroll
+    for (int m = 0; m < NM; ++m)
+        if (ok[m]) w[m] = load_at<V>(base, row_off + uint32_t(j[m]) * uint32_t(sizeof(V)));
+#pragma unroll
+    for (int m = 0; m < NM; ++m)
+        if (ok[m]) v[m] = cmul_sgn(v[m], w[m], sgn);
+}
+
+template <class V, int L1, int L2, int SIGN, int MODE, int SPEC>
 __global__ void __launch_bounds__(512) fft2_pass_kernel(PassArgs<V> a) {
     using R = decltype(V::x);
     constexpr int L = L1 * L2;
     constexpr int T = RegShape<L1, L2>::T;
     constexpr bool ROWS = MODE == PM_ROW_LOAD || MODE == PM_ROW_STORE;
     constexpr bool PSF = MODE == PM_COL_PSF;
+    constexpr bool HALF_IN = (SPEC & PS_HALF_IN) != 0;
+    constexpr bool HALF_OUT = (SPEC & PS_HALF_OUT) != 0;
+    constexpr uint32_t VB = sizeof(V);
+    // column passes stream their lines (nontemporal); the row load pass
+    // fetches the next batch element's input while transforming the current
+    // one (measured: the other passes gain nothing from it and lose occupancy)
+    constexpr bool NT = MODE == PM_COL || MODE == PM_COL_PSF;
+    constexpr bool PREFETCH = MODE == PM_ROW_LOAD && HALF_IN;
     // output layout: stride L1 (L2 values per owner) after a forward pass,
     // stride L2 (L1 values per owner) after the PSF round trip
     constexpr int NOWN = PSF ? L1 : L2;
@@ -222,32 +293,92 @@ __global__ void __launch_bounds__(512) fft2_pass_kernel(PassArgs<V> a) {
     const int64_t bblk = bid;
     const int i = it * TX + c;
     const bool line_ok = i < a.ni;
+    // loads of a tail line read the last valid line; nothing of it is stored
+    const int ic = line_ok ? i : a.ni - 1;
     V* xl = lds + c * a.ldp;   // this line's exchange area
-    // twiddle table W_L^k, k < L, staged once per block after the exchange areas
-    V* twl = lds + int64_t(TX) * a.ldp;
-    for (int k = tid; k < L; k += blockDim.x) twl[k] = a.tw[k];
+    // twiddles after the exchange areas, one row per step-1 owner so a
+    // thread's twiddles are consecutive (immediate LDS offsets):
+    //   twa[p2 * L1 + k1] = W_L^{p2 k1}  (forward step 1, p2 < L2)
+    //   twb[k1 * L2 + m]  = W_L^{k1 m}   (inverse step 1 of the PSF pass, k1 < L1)
+    V* twa = lds + int64_t(TX) * a.ldp;
+    V* twb = twa + L;
+    for (int k = tid; k < L; k += blockDim.x) {
+        twa[k] = a.tw[((k / L1) * (k % L1)) % L];
+        if (PSF) twb[k] = a.tw[((k / L2) * (k % L2)) % L];
+    }
     __syncthreads();
 
     const bool sum_b = MODE == PM_ROW_STORE && a.sum_batch;
-    // a PSF block runs bchunk consecutive batch elements of its lines, so
-    // each kernel value is read from HBM once per chunk, not once per element
-    const int64_t b_first = PSF ? bblk * a.bchunk : bblk;
-    const int nb_loop = sum_b ? a.nbatch
-                              : (PSF ? int(min(int64_t(a.bchunk), int64_t(a.nbatch) - b_first)) : 1);
-    // the kernel column of this thread's positions, the same for every
-    // batch element of the chunk: loaded once, issued together
+    // a block runs bchunk consecutive batch elements of its lines (a coil
+    // sum: all of them): the input of the next element is fetched while the
+    // current one is transformed, and per-line data that is the same for
+    // every element (the PSF kernel column, broadcast diagonals) is read once
+    const int64_t b_first = sum_b ? 0 : bblk * a.bchunk;
+    const int nb_loop = sum_b ? a.nbatch : int(min(int64_t(a.bchunk), int64_t(a.nbatch) - b_first));
     V kv[PSF ? L2 : 1];
     if constexpr (PSF) {
-        if (t < L1 && line_ok) {
-            const V* k = a.psf + int64_t(o) * a.psf_o + int64_t(i) * a.psf_i;
+        if (t < L1) {
+            const char* kb = reinterpret_cast<const char*>(a.psf + int64_t(o) * a.psf_o);
+            const uint32_t ko = uint32_t(ic) * a.psf_ib + uint32_t(t) * a.psf_sb;
 #pragma unroll
-            for (int k2 = 0; k2 < L2; ++k2) kv[k2] = k[int64_t(t + L1 * k2) * a.psf_s];
+            for (int k2 = 0; k2 < L2; ++k2) kv[k2] = load_at<V>(kb, ko + uint32_t(L1 * k2) * a.psf_sb);
         }
     }
     R zy = R(1);
-    if (ROWS && line_ok && a.dx) {
-        const int z = i / a.ny;
-        zy = a.dz[z] * a.dy[i - z * a.ny];
+    if (ROWS && a.dx) {
+        const int z = ic / a.ny;
+        zy = a.dz[z] * a.dy[ic - z * a.ny];
+    }
+    // step-1 input position p = L2 p1 + t of this thread -> line element j,
+    // valid when ok (shift and length passed in, laundered per iteration)
+    auto in_pos = [&](int p1, int shift, int len, int& j, bool& ok) {
+        const int p = L2 * p1 + t;
+        if constexpr (HALF_IN) {
+            // whole rows of p1 are known valid / zero; one mixed row (L1 odd)
+            // keeps a runtime test
+            j = p;
+            ok = t < L2 && ((L2 * p1 >= L / 2) ? false : (L2 * p1 + L2 <= L / 2 ? true : p < L / 2));
+        } else {
+            j = p - shift;
+            j = j < 0 ? j + L : (j >= L ? j - L : j);
+            ok = t < L2 && j < len;
+        }
+    };
+    // input of batch element bx (zeros outside the data)
+    auto fetch = [&](int64_t bx, V* dst) {
+        int icl = ic;
+        uint32_t in_sb = a.in_sb;
+        int shift = a.in_shift, len = a.in_len;
+        if (nb_loop > 1) asm volatile("" : "+v"(icl), "+s"(in_sb), "+s"(shift), "+s"(len));
+        const char* src = reinterpret_cast<const char*>(a.in + bx * a.in_b + int64_t(o) * a.in_o);
+        const uint32_t lo = uint32_t(icl) * a.in_ib;
+#pragma unroll
+        for (int p1 = 0; p1 < L1; ++p1) {
+            int j;
+            bool ok;
+            in_pos(p1, shift, len, j, ok);
+            dst[p1] = cmake<V>(R(0), R(0));
+            if (ok) dst[p1] = load_at<V, NT>(src, lo + uint32_t(j) * in_sb);
+        }
+    };
+    // broadcast load diagonals (batch stride 0) of the prefetching row pass
+    // are read once per block
+    V lw1[PREFETCH ? L1 : 1], lw2[PREFETCH ? L1 : 1];
+    if constexpr (PREFETCH) {
+        auto pre = [&](const DiagOp<V>& d, uint32_t rb, V* w) {
+            if (d.ptr && d.sb == 0) {
+                const char* base = reinterpret_cast<const char*>(d.ptr);
+#pragma unroll
+                for (int p1 = 0; p1 < L1; ++p1) {
+                    int j;
+                    bool ok;
+                    in_pos(p1, 0, L / 2, j, ok);
+                    if (ok) w[p1] = load_at<V>(base, uint32_t(ic) * rb + uint32_t(j) * VB);
+                }
+            }
+        };
+        pre(a.lm1, a.lm1_rb, lw1);
+        pre(a.lm2, a.lm2_rb, lw2);
     }
     // coil-sum accumulators: outputs j = t + T m, m < NACC (the coil sum is
     // used with a crop to at most L/2 outputs; the host checks out_len)
@@ -257,39 +388,59 @@ __global__ void __launch_bounds__(512) fft2_pass_kernel(PassArgs<V> a) {
 #pragma unroll
         for (int m = 0; m < NACC; ++m) acc[m] = cmake<V>(R(0), R(0));
     }
+    V nxt[PREFETCH ? L1 : 1];
+    if constexpr (PREFETCH) fetch(b_first, nxt);
     for (int bb = 0; bb < nb_loop; ++bb) {
-        const int64_t b = sum_b ? bb : b_first + bb;
+        const int64_t b = b_first + bb;
         // per-iteration copies the compiler cannot hoist: otherwise the
-        // per-position addresses and masks of every load and store stay live
-        // in registers across the batch loop (and halve the occupancy)
-        int64_t in_s = a.in_s, out_s = a.out_s;
-        int in_shift = a.in_shift, in_len = a.in_len, out_shift = a.out_shift, out_len = a.out_len;
+        // per-position offsets and masks of every load and store stay live in
+        // registers across the batch loop (and cut the occupancy)
+        uint32_t out_sb = a.out_sb;
+        int out_shift = a.out_shift, out_len = a.out_len, in_shift = a.in_shift, in_len = a.in_len;
+        int icl = ic;
         if (nb_loop > 1)
-            asm volatile("" : "+s"(in_s), "+s"(out_s), "+s"(in_shift), "+s"(in_len), "+s"(out_shift),
-                         "+s"(out_len));
+            asm volatile("" : "+s"(out_sb), "+s"(out_shift), "+s"(out_len), "+s"(in_shift), "+s"(in_len),
+                         "+v"(icl));
         V v[T];
-        // ---- load (step-1 layout: thread t = p2 holds p = L2 p1 + t) ----
-        if (t < L2) {
-            const V* src = a.in + b * a.in_b + int64_t(o) * a.in_o + int64_t(i) * a.in_i;
+        if constexpr (PREFETCH) {
 #pragma unroll
-            for (int p1 = 0; p1 < L1; ++p1) {
-                const int p = L2 * p1 + t;
-                int j = p - in_shift;
-                j = j < 0 ? j + L : (j >= L ? j - L : j);
-                V val = cmake<V>(R(0), R(0));
-                if (line_ok && j < in_len) {
-                    val = src[int64_t(j) * in_s];
-                    if constexpr (MODE == PM_ROW_LOAD) {
-                        if (a.lm1.ptr) val = diag_mult(val, a.lm1.ptr[b * a.lm1.sb + int64_t(i) * a.lm1.sr + j], a.lm1.type);
-                        if (a.lm2.ptr) val = diag_mult(val, a.lm2.ptr[b * a.lm2.sb + int64_t(i) * a.lm2.sr + j], a.lm2.type);
-                        if (a.dx) val = cscale(val, zy * a.dx[j]);
+            for (int p1 = 0; p1 < L1; ++p1) v[p1] = nxt[p1];
+            if (bb + 1 < nb_loop) fetch(b + 1, nxt);
+        } else {
+            fetch(b, v);
+        }
+        // ---- step 1: thread t = p2 holds p = L2 p1 + t ----
+        if (t < L2) {
+            if constexpr (MODE == PM_ROW_LOAD) {
+                int jin[L1];
+                bool okin[L1];
+#pragma unroll
+                for (int p1 = 0; p1 < L1; ++p1) in_pos(p1, in_shift, in_len, jin[p1], okin[p1]);
+                auto use = [&](const DiagOp<V>& d, uint32_t rb, const V* w) {
+                    if (!d.ptr) return;
+                    if constexpr (PREFETCH) {
+                        if (d.sb == 0) {
+                            const decltype(V::x) sgn = d.type == HASTY_MULT_CONJ ? -1 : 1;
+#pragma unroll
+                            for (int p1 = 0; p1 < L1; ++p1)
+                                if (okin[p1]) v[p1] = cmul_sgn(v[p1], w[p1], sgn);
+                            return;
+                        }
                     }
+                    apply_diag<V, L1>(v, d, b, uint32_t(icl) * rb, jin, okin);
+                };
+                use(a.lm1, a.lm1_rb, lw1);
+                use(a.lm2, a.lm2_rb, lw2);
+                if (a.dx) {
+#pragma unroll
+                    for (int p1 = 0; p1 < L1; ++p1)
+                        if (okin[p1]) v[p1] = cscale(v[p1], zy * a.dx[jin[p1]]);
                 }
-                v[p1] = val;
             }
             RegDFT<L1, SIGN, V>::run(v);
+            const V* tw = twa + t * L1;
 #pragma unroll
-            for (int k1 = 1; k1 < L1; ++k1) v[k1] = twiddle_mul<V, SIGN>(v[k1], twl, t * k1);
+            for (int k1 = 1; k1 < L1; ++k1) v[k1] = twiddle_mul<V, SIGN>(v[k1], tw, k1);
 #pragma unroll
             for (int k1 = 0; k1 < L1; ++k1) xl[t * (L1 + 1) + k1] = v[k1];
         }
@@ -302,14 +453,13 @@ __global__ void __launch_bounds__(512) fft2_pass_kernel(PassArgs<V> a) {
         }
         if constexpr (PSF) {
             if (t < L1) {
-                if (line_ok) {
 #pragma unroll
-                    for (int k2 = 0; k2 < L2; ++k2) v[k2] = cmul(v[k2], kv[k2]);
-                }
+                for (int k2 = 0; k2 < L2; ++k2) v[k2] = cmul(v[k2], kv[k2]);
                 // inverse, factorisation L = L2 * L1: step 1 over the L2 held values
                 RegDFT<L2, -SIGN, V>::run(v);
+                const V* tw = twb + t * L2;
 #pragma unroll
-                for (int k1 = 1; k1 < L2; ++k1) v[k1] = twiddle_mul<V, -SIGN>(v[k1], twl, t * k1);
+                for (int k1 = 1; k1 < L2; ++k1) v[k1] = twiddle_mul<V, -SIGN>(v[k1], tw, k1);
             }
             __syncthreads();   // step-2 reads of the first exchange are done
             if (t < L1) {
@@ -334,50 +484,81 @@ __global__ void __launch_bounds__(512) fft2_pass_kernel(PassArgs<V> a) {
             }
             __syncthreads();
             if (line_ok) {
-                V* dst = a.out + (sum_b ? 0 : b * a.out_b) + int64_t(o) * a.out_o + int64_t(i) * a.out_i;
-                const V* m1 = a.sm1.ptr ? a.sm1.ptr + b * a.sm1.sb + int64_t(i) * a.sm1.sr : nullptr;
-                const V* m2 = a.sm2.ptr ? a.sm2.ptr + b * a.sm2.sb + int64_t(i) * a.sm2.sr : nullptr;
-                const V* io = a.sio.ptr ? a.sio.ptr + b * a.sio.sb + int64_t(i) * a.sio.sr : nullptr;
-                if (sum_b) {
+                // outputs j = t + T m: HALF_OUT keeps j < L/2 at line positions
+                // q = j + L/2 - 1; otherwise j < out_len at q = j + out_shift mod L,
+                // taken in rounds of NACC outputs per thread (the coil sum has
+                // one round: the host keeps out_len <= NACC T)
+                constexpr int NR = HALF_OUT ? 1 : (L + T * NACC - 1) / (T * NACC);
+                char* dst = reinterpret_cast<char*>(a.out + (sum_b ? 0 : b * a.out_b) + int64_t(o) * a.out_o);
+                const uint32_t lo = uint32_t(icl) * a.out_ib;
+#pragma unroll
+                for (int r = 0; r < NR; ++r) {
+                    if (!HALF_OUT && T * NACC * r >= out_len) break;
+                    int jj[NACC];
+                    bool ok[NACC];
+                    V val[NACC];
 #pragma unroll
                     for (int m = 0; m < NACC; ++m) {
-                        const int j = t + T * m;
-                        if (j < out_len) {
-                            int q = j + out_shift;
+                        const int j = t + T * (NACC * r + m);
+                        int q;
+                        if constexpr (HALF_OUT) {
+                            ok[m] = T * (m + 1) <= L / 2 || j < L / 2;
+                            q = j + L / 2 - 1;
+                        } else {
+                            ok[m] = j < out_len;
+                            q = j + out_shift;
                             q = q >= L ? q - L : (q < 0 ? q + L : q);
-                            V val = cscale(xl[q], a.scale);
-                            if (m1) val = diag_mult(val, m1[j], a.sm1.type);
-                            if (m2) val = diag_mult(val, m2[j], a.sm2.type);
-                            if (io) val = diag_mult(val, io[j], a.sio.type);
-                            if (a.dx) val = cscale(val, zy * a.dx[j]);
-                            acc[m] = cadd(acc[m], val);
                         }
+                        jj[m] = j;
+                        val[m] = cmake<V>(R(0), R(0));
+                        if (ok[m]) val[m] = cscale(xl[q], a.scale);
                     }
-                } else {
-                    for (int j = t; j < out_len; j += T) {
-                        int q = j + out_shift;
-                        q = q >= L ? q - L : (q < 0 ? q + L : q);
-                        V val = cscale(xl[q], a.scale);
-                        if (m1) val = diag_mult(val, m1[j], a.sm1.type);
-                        if (m2) val = diag_mult(val, m2[j], a.sm2.type);
-                        if (io) val = diag_mult(val, io[j], a.sio.type);
-                        if (a.dx) val = cscale(val, zy * a.dx[j]);
-                        V* d = dst + j;
-                        if (a.accumulate) val = cadd(val, *d);
-                        *d = val;
+                    apply_diag<V, NACC>(val, a.sm1, b, uint32_t(icl) * a.sm1_rb, jj, ok);
+                    apply_diag<V, NACC>(val, a.sm2, b, uint32_t(icl) * a.sm2_rb, jj, ok);
+                    apply_diag<V, NACC>(val, a.sio, b, uint32_t(icl) * a.sio_rb, jj, ok);
+                    if (a.dx) {
+#pragma unroll
+                        for (int m = 0; m < NACC; ++m)
+                            if (ok[m]) val[m] = cscale(val[m], zy * a.dx[jj[m]]);
+                    }
+                    if (sum_b) {
+#pragma unroll
+                        for (int m = 0; m < NACC; ++m)
+                            if (ok[m]) acc[m] = cadd(acc[m], val[m]);
+                    } else {
+#pragma unroll
+                        for (int m = 0; m < NACC; ++m) {
+                            if (ok[m]) {
+                                const uint32_t off = lo + uint32_t(jj[m]) * VB;
+                                store_at<V>(dst, off, a.accumulate ? cadd(val[m], load_at<V>(dst, off)) : val[m]);
+                            }
+                        }
                     }
                 }
             }
         } else {
             // owner t holds positions t + OSTRIDE k, k < NOWN
             if (t < NOWNERS && line_ok) {
-                V* dst = a.out + b * a.out_b + int64_t(o) * a.out_o + int64_t(i) * a.out_i;
+                char* dst = reinterpret_cast<char*>(a.out + b * a.out_b + int64_t(o) * a.out_o);
+                const uint32_t lo = uint32_t(icl) * a.out_ib;
 #pragma unroll
                 for (int k = 0; k < NOWN; ++k) {
                     const int q = t + OSTRIDE * k;
-                    int j = q - out_shift;
-                    j = j < 0 ? j + L : (j >= L ? j - L : j);
-                    if (j < out_len) dst[int64_t(j) * out_s] = cscale(v[k], a.scale);
+                    int j;
+                    bool ok;
+                    if constexpr (HALF_OUT) {
+                        // kept: L/2 - 1 <= q < L - 1, q in [OSTRIDE k, OSTRIDE (k + 1))
+                        constexpr int lo_q = L / 2 - 1, hi_q = L - 1;
+                        j = q - lo_q;
+                        const bool none = OSTRIDE * (k + 1) <= lo_q || OSTRIDE * k >= hi_q;
+                        const bool all = OSTRIDE * k >= lo_q && OSTRIDE * (k + 1) <= hi_q;
+                        ok = none ? false : (all ? true : (q >= lo_q && q < hi_q));
+                    } else {
+                        j = q - out_shift;
+                        j = j < 0 ? j + L : (j >= L ? j - L : j);
+                        ok = j < out_len;
+                    }
+                    if (ok) store_at<V, NT>(dst, lo + uint32_t(j) * out_sb, cscale(v[k], a.scale));
                 }
             }
         }
@@ -385,14 +566,15 @@ __global__ void __launch_bounds__(512) fft2_pass_kernel(PassArgs<V> a) {
     }
     if constexpr (MODE == PM_ROW_STORE) {
         if (sum_b && line_ok) {
-            V* dst = a.out + int64_t(o) * a.out_o + int64_t(i) * a.out_i;
+            char* dst = reinterpret_cast<char*>(a.out + int64_t(o) * a.out_o);
+            const uint32_t lo = uint32_t(i) * a.out_ib;
+            const int out_len = HALF_OUT ? L / 2 : a.out_len;
 #pragma unroll
             for (int m = 0; m < NACC; ++m) {
                 const int j = t + T * m;
-                if (j < a.out_len) {
-                    V val = acc[m];
-                    if (a.accumulate) val = cadd(val, dst[j]);
-                    dst[j] = val;
+                if (j < out_len) {
+                    const uint32_t off = lo + uint32_t(j) * VB;
+                    store_at<V>(dst, off, a.accumulate ? cadd(acc[m], load_at<V>(dst, off)) : acc[m]);
                 }
             }
         }

@@ -12,16 +12,14 @@ namespace hasty {
 
 namespace {
 
-template <class V, int L1, int L2, int SIGN, int MODE>
+template <class V, int L1, int L2, int SIGN, int MODE, int SPEC>
 void launch_one(const PassArgs<V>& a, const char* tag, hipStream_t s) {
     constexpr int T = RegShape<L1, L2>::T;
-    const int64_t nb_blocks = (MODE == PM_ROW_STORE && a.sum_batch) ? 1
-                              : MODE == PM_COL_PSF                ? ceil_div(a.nbatch, a.bchunk)
-                                                                  : int64_t(a.nbatch);
+    const int64_t nb_blocks = (MODE == PM_ROW_STORE && a.sum_batch) ? 1 : ceil_div(a.nbatch, a.bchunk);
     const int64_t blocks = ceil_div(a.ni, a.tx) * int64_t(a.no) * nb_blocks;
     if (blocks == 0) return;
-    const size_t lds = (size_t(a.tx) * size_t(a.ldp) + size_t(L1 * L2)) * sizeof(V);
-    auto kern = fft2_pass_kernel<V, L1, L2, SIGN, MODE>;
+    const size_t lds = (size_t(a.tx) * size_t(a.ldp) + size_t(2 * L1 * L2)) * sizeof(V);
+    auto kern = fft2_pass_kernel<V, L1, L2, SIGN, MODE, SPEC>;
     if (lds > 64 * 1024) {
         static std::once_flag once;
         std::call_once(once, [&] {
@@ -35,18 +33,40 @@ void launch_one(const PassArgs<V>& a, const char* tag, hipStream_t s) {
     HASTY_CHECK_HIP(hipGetLastError());
 }
 
+// The half-line specialisations are built for the direction the Toeplitz
+// passes use them in (pad + forward transform, inverse transform + crop);
+// any other combination runs the general kernel.
 template <class V, int L1, int L2, int SIGN>
-void launch_mode(const PassArgs<V>& a, int mode, const char* tag, hipStream_t s) {
+void launch_mode(const PassArgs<V>& a, int mode, int spec, const char* tag, hipStream_t s) {
+    constexpr int F = -1, I = 1;   // forward / inverse signs
     switch (mode) {
-        case PM_COL: launch_one<V, L1, L2, SIGN, PM_COL>(a, tag, s); break;
-        case PM_COL_PSF: launch_one<V, L1, L2, SIGN, PM_COL_PSF>(a, tag, s); break;
-        case PM_ROW_LOAD: launch_one<V, L1, L2, SIGN, PM_ROW_LOAD>(a, tag, s); break;
-        default: launch_one<V, L1, L2, SIGN, PM_ROW_STORE>(a, tag, s); break;
+        case PM_COL:
+            if (SIGN == F && spec == PS_HALF_IN) return launch_one<V, L1, L2, SIGN, PM_COL, PS_HALF_IN>(a, tag, s);
+            if (SIGN == I && spec == PS_HALF_OUT) return launch_one<V, L1, L2, SIGN, PM_COL, PS_HALF_OUT>(a, tag, s);
+            return launch_one<V, L1, L2, SIGN, PM_COL, PS_NONE>(a, tag, s);
+        case PM_COL_PSF:
+            if (SIGN == F && spec == (PS_HALF_IN | PS_HALF_OUT))
+                return launch_one<V, L1, L2, SIGN, PM_COL_PSF, PS_HALF_IN | PS_HALF_OUT>(a, tag, s);
+            return launch_one<V, L1, L2, SIGN, PM_COL_PSF, PS_NONE>(a, tag, s);
+        case PM_ROW_LOAD:
+            if (SIGN == F && spec == PS_HALF_IN) return launch_one<V, L1, L2, SIGN, PM_ROW_LOAD, PS_HALF_IN>(a, tag, s);
+            return launch_one<V, L1, L2, SIGN, PM_ROW_LOAD, PS_NONE>(a, tag, s);
+        default:
+            if (SIGN == I && spec == PS_HALF_OUT) return launch_one<V, L1, L2, SIGN, PM_ROW_STORE, PS_HALF_OUT>(a, tag, s);
+            return launch_one<V, L1, L2, SIGN, PM_ROW_STORE, PS_NONE>(a, tag, s);
     }
 }
 
+// Largest byte offset a thread forms inside one (batch, o) slab: the kernel
+// addresses memory as a 64-bit slab base plus a 32-bit offset.
+inline bool offsets_fit_32(int64_t ni, int64_t line_stride, int64_t len, int64_t elem_stride, size_t vb) {
+    if (ni <= 0 || len <= 0) return true;
+    const int64_t maxoff = ((ni - 1) * line_stride + (len - 1) * elem_stride + 1) * int64_t(vb);
+    return line_stride >= 0 && elem_stride >= 0 && maxoff < (int64_t(1) << 32);
+}
+
 template <class V, int L1, int L2>
-void launch_shape(PassArgs<V> a, int sign, int mode, const char* tag, hipStream_t s) {
+bool launch_shape(PassArgs<V> a, int sign, int mode, const char* tag, hipStream_t s) {
     constexpr int T = RegShape<L1, L2>::T;
     constexpr int L = L1 * L2;
     a.ldp = L + T + 1;
@@ -54,8 +74,32 @@ void launch_shape(PassArgs<V> a, int sign, int mode, const char* tag, hipStream_
     HASTY_REQUIRE(!(mode == PM_ROW_STORE && a.sum_batch) ||
                       a.out_len <= ((L / 2 + T - 1) / T) * T,
                   "coil-sum pass supports outputs up to half the line");
+    const size_t vb = sizeof(V);
+    bool fit = offsets_fit_32(a.ni, a.in_i, a.in_len, a.in_s, vb) &&
+               offsets_fit_32(a.ni, a.out_i, a.out_len, a.out_s, vb);
+    if (a.psf) fit = fit && offsets_fit_32(a.ni, a.psf_i, L, a.psf_s, vb);
+    for (const DiagOp<V>* d : {&a.lm1, &a.lm2, &a.sm1, &a.sm2, &a.sio})
+        if (d->ptr) fit = fit && offsets_fit_32(a.ni, d->sr, L, 1, vb);
+    if (!fit) return false;   // the Stockham kernels take 64-bit offsets
+    a.in_ib = uint32_t(a.in_i * int64_t(vb));
+    a.in_sb = uint32_t(a.in_s * int64_t(vb));
+    a.out_ib = uint32_t(a.out_i * int64_t(vb));
+    a.out_sb = uint32_t(a.out_s * int64_t(vb));
+    a.psf_ib = uint32_t(a.psf_i * int64_t(vb));
+    a.psf_sb = uint32_t(a.psf_s * int64_t(vb));
+    a.lm1_rb = uint32_t(a.lm1.sr * int64_t(vb));
+    a.lm2_rb = uint32_t(a.lm2.sr * int64_t(vb));
+    a.sm1_rb = uint32_t(a.sm1.sr * int64_t(vb));
+    a.sm2_rb = uint32_t(a.sm2.sr * int64_t(vb));
+    a.sio_rb = uint32_t(a.sio.sr * int64_t(vb));
+    int spec = PS_NONE;
+    if (L % 2 == 0 && a.in_shift == 0 && a.in_len == L / 2) spec |= PS_HALF_IN;
+    if (L % 2 == 0 && a.out_shift == L / 2 - 1 && a.out_len == L / 2) spec |= PS_HALF_OUT;
     // HASTY_COL_TX / HASTY_PSF_TX: lines per block of the column passes
-    // (tuning; default 16 columns = 128 contiguous bytes of c64 per row)
+    // (tuning; defaults measured at 128^3 x 32 coils: 32 columns = 256
+    // contiguous bytes of c64 per row for the plain column passes, 16 for
+    // the PSF pass, whose larger register and LDS footprint per line would
+    // leave one 512-thread block per CU)
     static const int env_col_tx = std::getenv("HASTY_COL_TX") ? std::atoi(std::getenv("HASTY_COL_TX")) : 0;
     static const int env_psf_tx = std::getenv("HASTY_PSF_TX") ? std::atoi(std::getenv("HASTY_PSF_TX")) : 0;
     if (rows) {
@@ -65,21 +109,24 @@ void launch_shape(PassArgs<V> a, int sign, int mode, const char* tag, hipStream_
     } else if (env_col_tx > 0) {
         a.tx = env_col_tx;
     } else if (a.tx <= 0) {
-        a.tx = 16;
+        a.tx = mode == PM_COL_PSF ? 16 : 32;
     }
-    while (a.tx > 1 && (size_t(a.tx) * a.ldp + size_t(L)) * sizeof(V) > 160 * 1024) a.tx /= 2;
+    while (a.tx > 1 && (size_t(a.tx) * a.ldp + size_t(2 * L)) * sizeof(V) > 160 * 1024) a.tx /= 2;
     while (a.tx * T > 512) a.tx /= 2;
-    if (mode == PM_COL_PSF) {
-        // batch elements per block: as many as keep >= 4096 blocks in the
-        // launch, so the kernel columns are read from HBM once per chunk
+    // batch elements per block: the PSF pass and the half-line row load
+    // take as many as keep >= 4096 blocks in the launch (16 per CU) and read
+    // per-line data (the PSF kernel column, broadcast diagonals) once per
+    // chunk; the other passes run one element per block
+    if (!(mode == PM_COL_PSF || (mode == PM_ROW_LOAD && sign < 0 && (spec & PS_HALF_IN)))) {
+        a.bchunk = 1;
+    } else {
         const int64_t line_blocks = ceil_div(a.ni, a.tx) * int64_t(a.no);
         const int64_t nchunks = std::min<int64_t>(a.nbatch, std::max<int64_t>(1, ceil_div(4096, line_blocks)));
-        a.bchunk = int(ceil_div(a.nbatch, nchunks));
-    } else {
-        a.bchunk = 1;
+        a.bchunk = int(std::max<int64_t>(1, ceil_div(a.nbatch, nchunks)));
     }
-    if (sign < 0) launch_mode<V, L1, L2, -1>(a, mode, tag, s);
-    else launch_mode<V, L1, L2, 1>(a, mode, tag, s);
+    if (sign < 0) launch_mode<V, L1, L2, -1>(a, mode, spec, tag, s);
+    else launch_mode<V, L1, L2, 1>(a, mode, spec, tag, s);
+    return true;
 }
 
 }  // namespace
@@ -88,23 +135,23 @@ template <> bool launch_fft2_pass<HASTY_FFT2_V>(PassArgs<HASTY_FFT2_V> a, int L,
                                                  const char* tag, hipStream_t s) {
     using V = HASTY_FFT2_V;
     switch (L) {
-        case 16: launch_shape<V, 4, 4>(a, sign, rows, tag, s); return true;
-        case 20: launch_shape<V, 5, 4>(a, sign, rows, tag, s); return true;
-        case 24: launch_shape<V, 6, 4>(a, sign, rows, tag, s); return true;
-        case 32: launch_shape<V, 8, 4>(a, sign, rows, tag, s); return true;
-        case 40: launch_shape<V, 8, 5>(a, sign, rows, tag, s); return true;
-        case 48: launch_shape<V, 8, 6>(a, sign, rows, tag, s); return true;
-        case 64: launch_shape<V, 8, 8>(a, sign, rows, tag, s); return true;
-        case 80: launch_shape<V, 10, 8>(a, sign, rows, tag, s); return true;
-        case 96: launch_shape<V, 12, 8>(a, sign, rows, tag, s); return true;
-        case 128: launch_shape<V, 16, 8>(a, sign, rows, tag, s); return true;
-        case 160: launch_shape<V, 16, 10>(a, sign, rows, tag, s); return true;
-        case 192: launch_shape<V, 16, 12>(a, sign, rows, tag, s); return true;
-        case 256: launch_shape<V, 16, 16>(a, sign, rows, tag, s); return true;
-        case 320: launch_shape<V, 20, 16>(a, sign, rows, tag, s); return true;
-        case 384: launch_shape<V, 24, 16>(a, sign, rows, tag, s); return true;
-        case 512: launch_shape<V, 32, 16>(a, sign, rows, tag, s); return true;
-        case 640: launch_shape<V, 32, 20>(a, sign, rows, tag, s); return true;
+        case 16: return launch_shape<V, 4, 4>(a, sign, rows, tag, s);
+        case 20: return launch_shape<V, 5, 4>(a, sign, rows, tag, s);
+        case 24: return launch_shape<V, 6, 4>(a, sign, rows, tag, s);
+        case 32: return launch_shape<V, 8, 4>(a, sign, rows, tag, s);
+        case 40: return launch_shape<V, 8, 5>(a, sign, rows, tag, s);
+        case 48: return launch_shape<V, 8, 6>(a, sign, rows, tag, s);
+        case 64: return launch_shape<V, 8, 8>(a, sign, rows, tag, s);
+        case 80: return launch_shape<V, 10, 8>(a, sign, rows, tag, s);
+        case 96: return launch_shape<V, 12, 8>(a, sign, rows, tag, s);
+        case 128: return launch_shape<V, 16, 8>(a, sign, rows, tag, s);
+        case 160: return launch_shape<V, 16, 10>(a, sign, rows, tag, s);
+        case 192: return launch_shape<V, 16, 12>(a, sign, rows, tag, s);
+        case 256: return launch_shape<V, 16, 16>(a, sign, rows, tag, s);
+        case 320: return launch_shape<V, 20, 16>(a, sign, rows, tag, s);
+        case 384: return launch_shape<V, 24, 16>(a, sign, rows, tag, s);
+        case 512: return launch_shape<V, 32, 16>(a, sign, rows, tag, s);
+        case 640: return launch_shape<V, 32, 20>(a, sign, rows, tag, s);
         default: return false;
     }
 }

@@ -262,7 +262,9 @@ class SenseOffresonance:
     with S, cjl and bil are folded into the NUFFT's own load, interpolation
     and store passes.  bil ``[L, M]`` and cjl ``[L, N...]`` may be None (L = 1).
     ``coil_batch`` caps the coils per plan execution (0 = all); ``tol``
-    defaults to the reference's (1e-6 single, 1e-14 double precision).
+    defaults to 1e-6 in single precision (the reference's value) and 1e-13 in
+    double precision (the reference asks for 1e-14; FP64 plans here go down
+    to 1e-13, kernel width 14).
     """
 
     def __init__(self, coords: Sequence[torch.Tensor], smaps: torch.Tensor,
@@ -291,7 +293,7 @@ class SenseOffresonance:
         self.cdtype = cdtype
         nt = self.C if coil_batch <= 0 else min(int(coil_batch), self.C)
         if tol is None:
-            tol = 1e-6 if dtype == torch.float32 else 1e-14
+            tol = 1e-6 if dtype == torch.float32 else 1e-13
         mk = lambda sign, t: hfft.nufft_plan.make(  # noqa: E731
             hfft.nufft_opts(nmodes=self.n, ntransf=nt, tol=tol, sign=sign, **opt_kwargs), t, dtype)
         self.fwd = mk(hfft.nufft_sign.DEFAULT_TYPE_2, hfft.nufft_type.TYPE_2)
