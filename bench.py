@@ -113,9 +113,9 @@ def cpu_baseline(N, C, smaps_np, x_np, kernel_np, sample_coils):
 # two-level FFT kernel in its column-PSF mode, unique to P3), plus the
 # calibration kernel: torch.neg over 1 GiB reads and writes exactly 2^30 B.
 PMC_KERNELS = {
-    "toep_P3_z_psf": "fft2_pass_kernel<HIP_vector_type<float, 2u>, 16, 16, -1, 1>",
+    "toep_P3_z_psf": "fft2_pass_kernel<HIP_vector_type<float, 2u>, 16, 16, -1, 1, 3>",
     "nufft_spread": "spread_kernel<HIP_vector_type<float, 2u>, 32>",
-    "nufft_interp": "interp_kernel<HIP_vector_type<float, 2u>, 6>",
+    "nufft_interp": "interp_kernel<HIP_vector_type<float, 2u>, 6,",
 }
 CALIB_SUBSTR = "neg_kernel"
 CALIB_BYTES = 1 << 30

@@ -144,6 +144,7 @@ void launch_row_store(RowStoreArgs<V> a, hipStream_t s) {
         p.sm1 = a.m1; p.sm2 = a.m2; p.sio = a.io;
         p.dz = a.dz; p.dy = a.dy; p.dx = a.dx; p.ny = a.ny; p.dz_on_load = 0;
         p.accumulate = a.accumulate; p.sum_batch = a.sum_batch;
+        p.partial = a.scratch; p.partial_elems = a.scratch ? a.scratch_elems : 0;
         p.tw = a.tw;
         if (launch_fft2_pass<V>(p, a.L, a.sign, PM_ROW_STORE, a.tag ? a.tag : "row_store", s)) return;
     }

@@ -210,6 +210,10 @@ template <class V> struct RowStoreArgs {
     int ny;
     int accumulate;   // out += result
     int sum_batch;    // out has no batch axis; results summed over b
+    // optional device scratch (complex elements) free during this pass: lets
+    // a batch sum run in several block groups whose partial sums are added
+    // by a second kernel (deterministic order)
+    V* scratch; int64_t scratch_elems;
     FFTPlan1D plan; const V* tw;
     const char* tag;
 };

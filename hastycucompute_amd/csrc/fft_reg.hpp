@@ -173,6 +173,8 @@ template <class V> struct PassArgs {
     const V* psf; int64_t psf_o, psf_i, psf_s;
     int bchunk;                         // PSF passes: batch elements per block (kernel read once)
     int accumulate, sum_batch;
+    V* partial; int64_t partial_elems;  // scratch for a batch sum in block groups:
+    int groups;                         //   partial[g][row][j], g < groups
     const V* tw;                        // tw[k] = exp(-2 pi i k / L)
     int ldp;                            // LDS stride per line (complex elements)
     // byte strides as 32-bit values: within one (batch, o) slab every offset
@@ -309,12 +311,11 @@ __global__ void __launch_bounds__(512) fft2_pass_kernel(PassArgs<V> a) {
     __syncthreads();
 
     const bool sum_b = MODE == PM_ROW_STORE && a.sum_batch;
-    // a block runs bchunk consecutive batch elements of its lines (a coil
-    // sum: all of them): the input of the next element is fetched while the
-    // current one is transformed, and per-line data that is the same for
-    // every element (the PSF kernel column, broadcast diagonals) is read once
-    const int64_t b_first = sum_b ? 0 : bblk * a.bchunk;
-    const int nb_loop = sum_b ? a.nbatch : int(min(int64_t(a.bchunk), int64_t(a.nbatch) - b_first));
+    // a block runs bchunk consecutive batch elements of its lines (a batch
+    // sum: those of its group): per-line data that is the same for every
+    // element (the PSF kernel column, broadcast diagonals) is read once
+    const int64_t b_first = bblk * a.bchunk;
+    const int nb_loop = int(min(int64_t(a.bchunk), int64_t(a.nbatch) - b_first));
     V kv[PSF ? L2 : 1];
     if constexpr (PSF) {
         if (t < L1) {
@@ -566,18 +567,45 @@ __global__ void __launch_bounds__(512) fft2_pass_kernel(PassArgs<V> a) {
     }
     if constexpr (MODE == PM_ROW_STORE) {
         if (sum_b && line_ok) {
-            char* dst = reinterpret_cast<char*>(a.out + int64_t(o) * a.out_o);
-            const uint32_t lo = uint32_t(i) * a.out_ib;
             const int out_len = HALF_OUT ? L / 2 : a.out_len;
+            if (a.groups > 1) {
+                // this group's partial sum, rows of out_len values
+                V* part = a.partial + (bblk * a.ni + i) * int64_t(out_len);
 #pragma unroll
-            for (int m = 0; m < NACC; ++m) {
-                const int j = t + T * m;
-                if (j < out_len) {
-                    const uint32_t off = lo + uint32_t(j) * VB;
-                    store_at<V>(dst, off, a.accumulate ? cadd(acc[m], load_at<V>(dst, off)) : acc[m]);
+                for (int m = 0; m < NACC; ++m) {
+                    const int j = t + T * m;
+                    if (j < out_len) part[j] = acc[m];
+                }
+            } else {
+                char* dst = reinterpret_cast<char*>(a.out + int64_t(o) * a.out_o);
+                const uint32_t lo = uint32_t(i) * a.out_ib;
+#pragma unroll
+                for (int m = 0; m < NACC; ++m) {
+                    const int j = t + T * m;
+                    if (j < out_len) {
+                        const uint32_t off = lo + uint32_t(j) * VB;
+                        store_at<V>(dst, off, a.accumulate ? cadd(acc[m], load_at<V>(dst, off)) : acc[m]);
+                    }
                 }
             }
         }
+    }
+}
+
+// out[row][j] (+)= sum_g partial[g][row][j], groups summed in order
+template <class V>
+__global__ void __launch_bounds__(256) sum_partials_kernel(const V* __restrict__ partial, V* out, int64_t nrows,
+                                                           int out_len, int64_t out_r, int groups,
+                                                           int accumulate) {
+    const int64_t n = nrows * out_len;
+    for (int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < n;
+         idx += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t r = idx / out_len;
+        const int j = int(idx - r * out_len);
+        V s = partial[idx];
+        for (int g = 1; g < groups; ++g) s = cadd(s, partial[g * n + idx]);
+        V* d = out + r * out_r + j;
+        *d = accumulate ? cadd(s, *d) : s;
     }
 }
 

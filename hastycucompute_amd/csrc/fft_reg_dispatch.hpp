@@ -15,7 +15,7 @@ namespace {
 template <class V, int L1, int L2, int SIGN, int MODE, int SPEC>
 void launch_one(const PassArgs<V>& a, const char* tag, hipStream_t s) {
     constexpr int T = RegShape<L1, L2>::T;
-    const int64_t nb_blocks = (MODE == PM_ROW_STORE && a.sum_batch) ? 1 : ceil_div(a.nbatch, a.bchunk);
+    const int64_t nb_blocks = ceil_div(a.nbatch, a.bchunk);
     const int64_t blocks = ceil_div(a.ni, a.tx) * int64_t(a.no) * nb_blocks;
     if (blocks == 0) return;
     const size_t lds = (size_t(a.tx) * size_t(a.ldp) + size_t(2 * L1 * L2)) * sizeof(V);
@@ -31,6 +31,13 @@ void launch_one(const PassArgs<V>& a, const char* tag, hipStream_t s) {
     HASTY_PROF(tag, s);
     hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(unsigned(a.tx * T)), lds, s, a);
     HASTY_CHECK_HIP(hipGetLastError());
+    if (MODE == PM_ROW_STORE && a.sum_batch && a.groups > 1) {
+        const int64_t n = a.ni * int64_t(a.out_len);
+        hipLaunchKernelGGL(sum_partials_kernel<V>, dim3(unsigned(std::min<int64_t>(ceil_div(n, 256), 4096))),
+                           dim3(256), 0, s, static_cast<const V*>(a.partial), a.out, int64_t(a.ni), a.out_len,
+                           a.out_i, a.groups, a.accumulate);
+        HASTY_CHECK_HIP(hipGetLastError());
+    }
 }
 
 // The half-line specialisations are built for the direction the Toeplitz
@@ -102,6 +109,8 @@ bool launch_shape(PassArgs<V> a, int sign, int mode, const char* tag, hipStream_
     // leave one 512-thread block per CU)
     static const int env_col_tx = std::getenv("HASTY_COL_TX") ? std::atoi(std::getenv("HASTY_COL_TX")) : 0;
     static const int env_psf_tx = std::getenv("HASTY_PSF_TX") ? std::atoi(std::getenv("HASTY_PSF_TX")) : 0;
+    // HASTY_PSF_BLOCKS: block count the PSF / row-load batch chunks aim for (tuning)
+    static const int64_t env_blocks = std::getenv("HASTY_PSF_BLOCKS") ? std::atoll(std::getenv("HASTY_PSF_BLOCKS")) : 0;
     if (rows) {
         a.tx = std::max(1, 256 / T);
     } else if (mode == PM_COL_PSF && env_psf_tx > 0) {
@@ -117,11 +126,27 @@ bool launch_shape(PassArgs<V> a, int sign, int mode, const char* tag, hipStream_
     // take as many as keep >= 4096 blocks in the launch (16 per CU) and read
     // per-line data (the PSF kernel column, broadcast diagonals) once per
     // chunk; the other passes run one element per block
-    if (!(mode == PM_COL_PSF || (mode == PM_ROW_LOAD && sign < 0 && (spec & PS_HALF_IN)))) {
+    a.groups = 1;
+    if (mode == PM_ROW_STORE && a.sum_batch) {
+        // a batch sum: each block sums the elements of one group of its rows;
+        // several groups (partials in the scratch, added by a second kernel)
+        // when one block per row tile would leave the GPU under-filled
+        const int64_t row_blocks = ceil_div(a.ni, a.tx);
+        int64_t groups = std::min<int64_t>(a.nbatch, std::max<int64_t>(1, ceil_div(2048, row_blocks)));
+        const int64_t per = ceil_div(a.nbatch, groups);
+        groups = ceil_div(a.nbatch, per);
+        if (groups > 1 && a.partial && groups * a.ni * int64_t(a.out_len) <= a.partial_elems) {
+            a.groups = int(groups);
+            a.bchunk = int(per);
+        } else {
+            a.bchunk = a.nbatch;
+        }
+    } else if (!(mode == PM_COL_PSF || (mode == PM_ROW_LOAD && sign < 0 && (spec & PS_HALF_IN)))) {
         a.bchunk = 1;
     } else {
         const int64_t line_blocks = ceil_div(a.ni, a.tx) * int64_t(a.no);
-        const int64_t nchunks = std::min<int64_t>(a.nbatch, std::max<int64_t>(1, ceil_div(4096, line_blocks)));
+        const int64_t target = env_blocks > 0 ? env_blocks : 4096;
+        const int64_t nchunks = std::min<int64_t>(a.nbatch, std::max<int64_t>(1, ceil_div(target, line_blocks)));
         a.bchunk = int(std::max<int64_t>(1, ceil_div(a.nbatch, nchunks)));
     }
     if (sign < 0) launch_mode<V, L1, L2, -1>(a, mode, spec, tag, s);

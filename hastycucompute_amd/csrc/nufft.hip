@@ -120,6 +120,8 @@ void pruned_inverse(const PadGeometry& g, int64_t B, const V* grid, V* img, V* b
     r.dz = deapod ? deapod[0] : nullptr; r.dy = deapod ? deapod[1] : nullptr;
     r.dx = deapod ? deapod[2] : nullptr; r.ny = int(ny);
     r.accumulate = accumulate; r.sum_batch = sum_batch;
+    // in 3-D bufB is free once the y pass has read it (scratch for a batch sum)
+    r.scratch = Lz > 1 ? bufB : nullptr; r.scratch_elems = B * nz * Ly * Lx;
     r.plan = make_fft_plan(int(Lx));
     r.tw = reinterpret_cast<const V*>(twiddles(int(Lx), sizeof(V) == 16));
     r.tag = "inv_x_fft_crop";

@@ -110,6 +110,7 @@ static void toeplitz_core(const ToeplitzDims& d, int64_t B, const float2* input,
     w.m1 = out1; w.m2 = out2; w.io = io;
     w.dz = w.dy = w.dx = nullptr; w.ny = int(ny);
     w.accumulate = accumulate; w.sum_batch = sum_batch;
+    w.scratch = bufB; w.scratch_elems = B * d.nB();   // bufB is free once P4 has read it
     w.plan = make_fft_plan(int(Lx));
     w.tw = reinterpret_cast<const float2*>(twiddles(int(Lx), false));
     w.tag = "toep_P5_x_ifft_crop";

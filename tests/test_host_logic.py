@@ -115,3 +115,66 @@ def test_autotune_candidates_and_options():
     opts = autotune.plan_options({"tile": [8, 8, 16], "max_subproblem": 1024, "ms": 1.0, "trials": []})
     assert opts == {"tile": (8, 8, 16), "max_subproblem": 1024}
     hf.nufft_opts(nmodes=(8, 8, 8), **opts)      # accepted by the options block
+
+
+# ---------------------------------------------------------------------------
+# time-segmentation interpolators (hastycucompute_amd.offres) vs the oracle
+# ---------------------------------------------------------------------------
+def _fmap_on_grid(rng, shape, values):
+    return np.asarray(values)[rng.integers(0, len(values), size=shape)]
+
+
+def test_offres_exact_for_few_values():
+    """A map with L distinct values on the bin centres is fitted exactly."""
+    import torch
+    from hastycucompute_amd.offres import make_offresonance_interpolators
+    from oracle.offres import interpolators
+    rng = np.random.default_rng(0)
+    nb = 11
+    grid = np.linspace(-300.0, 200.0, nb)
+    vals = 1j * grid[[0, 4, 10]]                       # three frequencies (rad/s), on centres
+    fmap = _fmap_on_grid(rng, (6, 5, 4), vals)
+    t = np.sort(rng.uniform(0.0, 8e-3, 300))
+    B, Ct = make_offresonance_interpolators(torch.from_numpy(fmap), torch.from_numpy(t), 3, nb, b0_only=True,
+                                            dtype=torch.complex128)
+    E = np.exp(fmap.reshape(-1)[None, :] * t[:, None])
+    approx = B.numpy().T @ Ct.numpy().reshape(3, -1)
+    assert np.linalg.norm(approx - E) / np.linalg.norm(E) < 1e-9
+    Bo, Cto = interpolators(fmap, t, 3, nb, b0_only=True)
+    assert np.abs(B.numpy() - Bo).max() < 1e-8 * np.abs(Bo).max()
+    assert np.abs(Ct.numpy() - Cto).max() < 1e-12
+
+
+def test_offres_error_falls_with_segments_and_matches_oracle():
+    import torch
+    from hastycucompute_amd.offres import make_offresonance_interpolators
+    from oracle.offres import interpolators
+    rng = np.random.default_rng(1)
+    shape = (8, 8, 8)
+    zz, yy, xx = np.meshgrid(*[np.linspace(-1, 1, n) for n in shape], indexing="ij")
+    fmap = -20.0 + 1j * 2 * np.pi * 60.0 * (xx ** 2 + 0.5 * yy - 0.3 * zz)   # R2* decay + smooth field
+    t = np.linspace(0.0, 10e-3, 500)
+    E = np.exp(fmap.reshape(-1)[None, :] * t[:, None])
+    errs = []
+    for L in (2, 4, 8):
+        B, Ct = make_offresonance_interpolators(torch.from_numpy(fmap), torch.from_numpy(t), L, 40,
+                                                dtype=torch.complex128)
+        Bo, Cto = interpolators(fmap, t, L, 40)
+        assert np.linalg.norm(B.numpy() - Bo) <= 1e-6 * np.linalg.norm(Bo)
+        approx = B.numpy().T @ Ct.numpy().reshape(L, -1)
+        errs.append(np.linalg.norm(approx - E) / np.linalg.norm(E))
+    assert errs[0] > errs[1] > errs[2] and errs[2] < 1e-2, errs
+
+
+def test_offres_autocorrelated_histogram():
+    import torch
+    from hastycucompute_amd.offres import histogram
+    from oracle.offres import histogram as ohist
+    rng = np.random.default_rng(2)
+    fmap = 1j * rng.uniform(-50, 50, (5, 6, 7))
+    zk, h = histogram(torch.from_numpy(fmap), 16, b0_only=True, autocorrelate=True)
+    zko, ho = ohist(fmap, 16, b0_only=True, autocorrelate=True)
+    assert h.numel() == 31 and abs(float(h.sum()) - fmap.size ** 2) < 1e-6
+    assert np.allclose(h.numpy(), ho) and np.allclose(zk.numpy(), zko)
+    span = fmap.imag.max() - fmap.imag.min()
+    assert np.isclose(zk.imag.max().item(), span) and np.isclose(zk.imag.min().item(), -span)
