@@ -34,6 +34,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdint>
+#include <fstream>
 #include <functional>
 #include <memory>
 #include <numeric>
@@ -161,6 +162,29 @@ private:
     std::array<int64_t, RANK> _shape{};
     std::vector<TT> _data;
 };
+
+// ---- raw tensor files (util_io.cppm:47-84; the `.htc` cache blocks of
+// mod_tensor_caching.cppm:84-96): the elements' bytes and nothing else, so
+// the reader supplies the shape and element type.  A short file is an error
+// here (the reference reads what is there).
+template <class TT, size_t RANK>
+void export_binary_tensor(const tensor<cpu_t, TT, RANK>& t, const std::string& path) {
+    std::ofstream file(path, std::ios::binary);
+    if (!file.is_open()) throw std::runtime_error("Could not open file for writing");
+    file.write(reinterpret_cast<const char*>(t.data_ptr()), std::streamsize(t.numel() * int64_t(sizeof(TT))));
+    if (!file) throw std::runtime_error("Exception opening/reading file");
+}
+
+template <class TT, size_t RANK>
+tensor<cpu_t, TT, RANK> import_binary_tensor(const std::string& path, std::array<int64_t, RANK> sizes) {
+    tensor<cpu_t, TT, RANK> t(sizes);
+    std::ifstream file(path, std::ios::binary | std::ios::in);
+    if (!file.is_open()) throw std::runtime_error("Could not open file for reading");
+    const std::streamsize want = std::streamsize(t.numel() * int64_t(sizeof(TT)));
+    file.read(reinterpret_cast<char*>(t.mutable_data_ptr()), want);
+    if (file.gcount() != want) throw std::runtime_error("Exception opening/reading file");
+    return t;
+}
 
 template <class TT, size_t RANK>
 tensor<cuda_t, TT, RANK> make_empty_tensor(std::array<int64_t, RANK> shape) {

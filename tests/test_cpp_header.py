@@ -22,3 +22,38 @@ def test_cpp_test_program_links():
         build.build()
     path = build.build_cpp_test()
     assert os.access(path, os.X_OK)
+
+
+def test_binary_tensor_io_host_only(tmp_path):
+    """export/import_binary_tensor of the C++ mirror: host-only round trip,
+    raw bytes (no header) readable by the Python mirror, and its errors."""
+    src = tmp_path / "io.cpp"
+    out = tmp_path / "blk.htc"
+    src.write_text(r'''
+#include <hasty/corelib.hpp>
+#include <cstdio>
+using namespace hasty;
+int main(int argc, char** argv) {
+    tensor<cpu_t, c64_t, 2> t({3, 4});
+    for (int i = 0; i < 12; ++i) t.vec()[size_t(i)] = c64_t(float(i), -0.5f * float(i));
+    export_binary_tensor(t, argv[1]);
+    auto u = import_binary_tensor<c64_t, 2>(argv[1], {3, 4});
+    for (int i = 0; i < 12; ++i) if (u.vec()[size_t(i)] != t.vec()[size_t(i)]) return 2;
+    try { (void)import_binary_tensor<c64_t, 2>(argv[1], {4, 4}); return 3; }
+    catch (const std::runtime_error& e) { if (std::string(e.what()) != "Exception opening/reading file") return 4; }
+    try { (void)import_binary_tensor<c64_t, 1>(std::string(argv[1]) + ".none", {1}); return 5; }
+    catch (const std::runtime_error& e) { if (std::string(e.what()) != "Could not open file for reading") return 6; }
+    return 0;
+}
+''')
+    exe = tmp_path / "io"
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(ROOT, "include"),
+                        "-I", "/opt/rocm/include", str(src), "-o", str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe), str(out)], capture_output=True, text=True)
+    assert r.returncode == 0, (r.returncode, r.stderr)
+    import torch
+    from hastycucompute_amd.io import import_binary_tensor
+    t = import_binary_tensor(out, (3, 4), torch.complex64)
+    ref = torch.complex(torch.arange(12.0), -0.5 * torch.arange(12.0)).reshape(3, 4)
+    assert torch.equal(t, ref)

@@ -178,3 +178,23 @@ def test_offres_autocorrelated_histogram():
     assert np.allclose(h.numpy(), ho) and np.allclose(zk.numpy(), zko)
     span = fmap.imag.max() - fmap.imag.min()
     assert np.isclose(zk.imag.max().item(), span) and np.isclose(zk.imag.min().item(), -span)
+
+
+def test_binary_tensor_round_trip(tmp_path):
+    import torch
+    from hastycucompute_amd.io import export_binary_tensor, import_binary_tensor
+    g = torch.Generator().manual_seed(0)
+    for dt, shape in ((torch.complex64, (3, 4, 5)), (torch.float32, (7,)), (torch.complex128, (2, 3))):
+        x = torch.randn(shape, dtype=dt, generator=g)
+        p = tmp_path / f"blk_{str(dt)[6:]}.htc"
+        export_binary_tensor(x, p)
+        assert p.stat().st_size == x.numel() * x.element_size()   # raw bytes, no header
+        y = import_binary_tensor(p, shape, dt)
+        assert torch.equal(x, y)
+    # the bytes are the C-order elements: a reinterpretation with another shape sees them in order
+    z = import_binary_tensor(tmp_path / "blk_float32.htc", (7,), torch.float32)
+    assert z.shape == (7,)
+    with pytest.raises(RuntimeError, match="Exception opening/reading file"):
+        import_binary_tensor(p, (4, 3), torch.complex128)
+    with pytest.raises(RuntimeError, match="Could not open file for reading"):
+        import_binary_tensor(tmp_path / "missing.htc", (1,), torch.float32)
