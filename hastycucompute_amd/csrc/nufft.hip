@@ -1,6 +1,7 @@
 // Type-1 / type-2 NUFFT for gfx950: bin sort, LDS-tiled spread and interp,
 // pruned Stockham FFT passes with fused deapodisation.
 #include <algorithm>
+#include <cstdlib>
 #include <memory>
 #include <type_traits>
 #include <cmath>
@@ -698,6 +699,19 @@ template <class V> using GridFn = void (*)(GridArgs<V>);
 template <class V>
 using SpreadFn = void (*)(GridArgs<V>, const int4*, const V*, const decltype(V::x)*);
 
+// Consecutive subproblems per XCD run of the spread / interpolation grids.
+// Measured at 128^3 x 32 coils x 8.4M points: spreading gains from runs of
+// 32 tiles (a row of tiles along x shares its straddling points in one L2:
+// 28.0 -> 24.7 ms); interpolation is fastest with single subproblems
+// interleaved over the XCDs.  HASTY_XCD_CHUNK overrides both (tuning).
+static int xcd_chunk(bool spread) {
+    static const int env = [] {
+        const char* e = std::getenv("HASTY_XCD_CHUNK");
+        return e ? std::max(1, std::atoi(e)) : 0;
+    }();
+    return env > 0 ? env : (spread ? 32 : 1);
+}
+
 template <class V>
 void execute_impl(NufftPlan* p, V* c, V* f, const NufftHooks& hk, int ntransf) {
     using R = decltype(V::x);
@@ -772,10 +786,10 @@ void execute_impl(NufftPlan* p, V* c, V* f, const NufftHooks& hk, int ntransf) {
                 ga.subs = p->sub_interp;
                 ga.nsub = p->nsub_interp;
                 ga.ngroups = int(ngroups);
+                ga.xchunk = xcd_chunk(false);
                 HASTY_PROF("nufft_interp", s);
-                // XCD-aware numbering (xcd_block): a 1-D grid of
-                // ceil(nsub / 8) * 8 * ngroups ids
-                const int64_t nblocks = ceil_div(p->nsub_interp, kXcds) * kXcds * int64_t(ngroups);
+                // XCD-aware numbering (xcd_block) over a 1-D grid
+                const int64_t nblocks = xcd_grid(p->nsub_interp, ngroups, ga.xchunk);
                 hipLaunchKernelGGL(reinterpret_cast<GridFn<V>>(interp_ptr(p)), dim3(unsigned(nblocks)),
                                    dim3(kNufftThreads), interp_lds_bytes(p, ga.group), s, ga);
             }
@@ -805,9 +819,13 @@ void execute_impl(NufftPlan* p, V* c, V* f, const NufftHooks& hk, int ntransf) {
                     const int b0 = p->T[0] == 1 ? 1 : 4, b1 = p->T[0] == 1 ? 8 : 4, b2 = b1;
                     const int64_t nsb = ceil_div(p->T[0], b0) * ceil_div(p->T[1], b1) * ceil_div(p->T[2], b2);
                     const unsigned threads = unsigned(64 * std::min<int64_t>(8, nsb));
-                    const unsigned ng = unsigned(ceil_div(nb, p->group));
+                    const int ng = int(ceil_div(nb, p->group));
+                    ga.nsub = p->nsub_spread;
+                    ga.ngroups = ng;
+                    ga.xchunk = xcd_chunk(true);
                     void* fn = p->f64 ? nufft_spread_ptr<double2>(p->group) : nufft_spread_ptr<float2>(p->group);
-                    hipLaunchKernelGGL(reinterpret_cast<SpreadFn<V>>(fn), dim3(unsigned(p->nsub_spread), ng),
+                    hipLaunchKernelGGL(reinterpret_cast<SpreadFn<V>>(fn),
+                                       dim3(unsigned(xcd_grid(p->nsub_spread, ng, ga.xchunk))),
                                        dim3(threads), 0, s, ga, static_cast<const int4*>(p->visits),
                                        static_cast<const V*>(p->csort), static_cast<const R*>(p->sw));
                     HASTY_CHECK_HIP(hipGetLastError());

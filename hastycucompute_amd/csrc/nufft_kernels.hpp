@@ -45,7 +45,8 @@ struct GridArgs {
     int64_t M;
     int nbatch, group, w, ws;
     V* partial;             // [slot][nbatch][tile cells]: partial tiles of split tiles
-    int nsub, ngroups;      // interpolation: subproblems and transform groups (xcd_block)
+    int nsub, ngroups;      // subproblems and transform groups (xcd_block)
+    int xchunk;             // consecutive subproblems per XCD run (xcd_block)
     const V* pw;            // optional per-point factor [M], caller order (SENSE time segments)
     int accumulate;         // interpolation: add into cdata
     Geo g;
@@ -61,20 +62,28 @@ __device__ inline void store_point(const GridArgs<V>& a, int64_t row, int orig, 
     *dst = v;
 }
 
-// Interpolation blocks are numbered so that the transform groups of one
-// subproblem are consecutive in dispatch order on one XCD: workgroups go to
-// the 8 XCDs round-robin by id, so with id = ((sub / 8) * ngroups + group) * 8
-// + sub % 8 every group of `sub` lands on XCD sub % 8, one after the other,
-// and the subproblem's point data (weights, indices) is read from HBM about
-// once and from that XCD's L2 for the other groups.  Returns false for the
-// padding ids past the last subproblem.
+// Spread / interpolation blocks are numbered with the 8 XCDs in mind:
+// workgroups go to the XCDs round-robin by id, so XCD x runs ids x, x + 8,
+// x + 16, ...  Here its k-th block takes transform group k % ngroups of
+// subproblem ((r * 8 + x) * chunk + j), with r, j = divmod(k / ngroups, chunk):
+// each XCD works through runs of `chunk` consecutive subproblems (tiles
+// that are neighbours along x, whose halos and straddling points it then
+// reads from its own L2), all transform groups of a subproblem one after the
+// other, and the runs alternate between XCDs so the work stays balanced.
+// Returns false for the padding ids past the last subproblem.
 constexpr int kXcds = 8;
-__device__ inline bool xcd_block(int nsub, int ngroups, int& sub, int& group) {
+__device__ inline bool xcd_block(int nsub, int ngroups, int chunk, int& sub, int& group) {
     const int id = int(blockIdx.x);
-    const int r = id % kXcds, q = id / kXcds;
-    group = q % ngroups;
-    sub = (q / ngroups) * kXcds + r;
+    const int x = id % kXcds, k = id / kXcds;
+    group = k % ngroups;
+    const int q = k / ngroups;
+    sub = ((q / chunk) * kXcds + x) * chunk + q % chunk;
     return sub < nsub;
+}
+// number of ids the mapping above needs for nsub subproblems
+__host__ inline int64_t xcd_grid(int64_t nsub, int64_t ngroups, int64_t chunk) {
+    const int64_t per_round = kXcds * chunk;
+    return ((nsub + per_round - 1) / per_round) * per_round * ngroups;
 }
 
 // ---------------------------------------------------------------------------
@@ -91,7 +100,7 @@ __global__ void __launch_bounds__(kNufftThreads) interp_kernel(GridArgs<V> a) {
     extern __shared__ unsigned char smem_raw[];
     V* lds = reinterpret_cast<V*>(smem_raw);
     int isub, igroup;
-    if (!xcd_block(a.nsub, a.ngroups, isub, igroup)) return;
+    if (!xcd_block(a.nsub, a.ngroups, a.xchunk, isub, igroup)) return;
     const int4 sub = a.subs[isub];
     int t0[3];
     tile_origin(a.g, sub.x, t0);
@@ -211,7 +220,9 @@ __global__ void __launch_bounds__(kNufftThreads)
     spread_kernel(GridArgs<V> a, const int4* __restrict__ visits, const V* __restrict__ csort,
                   const decltype(V::x)* __restrict__ sw) {
     using R = decltype(V::x);
-    const int4 sub = a.subs[blockIdx.x];
+    int isub, igroup;
+    if (!xcd_block(a.nsub, a.ngroups, a.xchunk, isub, igroup)) return;
+    const int4 sub = a.subs[isub];
     int t0[3];
     tile_origin(a.g, sub.x, t0);
     const int T0 = a.g.T[0], T1 = a.g.T[1], T2 = a.g.T[2];
@@ -221,7 +232,7 @@ __global__ void __launch_bounds__(kNufftThreads)
     const int B0 = T0 == 1 ? 1 : 4, B1 = T0 == 1 ? 8 : 4, B2 = T0 == 1 ? 8 : 4;
     const int ns1 = (T1 + B1 - 1) / B1, ns2 = (T2 + B2 - 1) / B2;
     const int nsb = ((T0 + B0 - 1) / B0) * ns1 * ns2;
-    const int g0 = blockIdx.y * CG;
+    const int g0 = igroup * CG;
     const int ng = min(CG, a.nbatch - g0);
     const int lane = threadIdx.x & 63;
     const int c2 = lane % B2, c1 = (lane / B2) % B1, c0 = lane / (B2 * B1);
