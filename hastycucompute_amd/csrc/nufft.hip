@@ -818,7 +818,12 @@ void execute_impl(NufftPlan* p, V* c, V* f, const NufftHooks& hk, int ntransf) {
                     // one wave per 64-cell sub-block of the tile, up to 8 per block
                     const int b0 = p->T[0] == 1 ? 1 : 4, b1 = p->T[0] == 1 ? 8 : 4, b2 = b1;
                     const int64_t nsb = ceil_div(p->T[0], b0) * ceil_div(p->T[1], b1) * ceil_div(p->T[2], b2);
-                    const unsigned threads = unsigned(64 * std::min<int64_t>(8, nsb));
+                    // waves per block: one per 64-cell sub-block, up to 8, and as many as
+                    // keep each wave's staged weight rows (64 x ws) within 64 KB of LDS
+                    const int64_t row_bytes = int64_t(64) * weight_stride(p->w) * int64_t(sizeof(R));
+                    const int64_t nw = std::max<int64_t>(1, std::min<int64_t>({8, nsb, (64 * 1024) / row_bytes}));
+                    const unsigned threads = unsigned(64 * nw);
+                    const size_t lds = size_t(nw * row_bytes);
                     const int ng = int(ceil_div(nb, p->group));
                     ga.nsub = p->nsub_spread;
                     ga.ngroups = ng;
@@ -826,7 +831,7 @@ void execute_impl(NufftPlan* p, V* c, V* f, const NufftHooks& hk, int ntransf) {
                     void* fn = p->f64 ? nufft_spread_ptr<double2>(p->group) : nufft_spread_ptr<float2>(p->group);
                     hipLaunchKernelGGL(reinterpret_cast<SpreadFn<V>>(fn),
                                        dim3(unsigned(xcd_grid(p->nsub_spread, ng, ga.xchunk))),
-                                       dim3(threads), 0, s, ga, static_cast<const int4*>(p->visits),
+                                       dim3(threads), lds, s, ga, static_cast<const int4*>(p->visits),
                                        static_cast<const V*>(p->csort), static_cast<const R*>(p->sw));
                     HASTY_CHECK_HIP(hipGetLastError());
                 }

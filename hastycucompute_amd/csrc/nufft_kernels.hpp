@@ -240,6 +240,12 @@ __global__ void __launch_bounds__(kNufftThreads)
     const int nwaves = blockDim.x >> 6;
     const int W = a.w;
     const int64_t G = a.g.nf[0] * a.g.nf[1] * a.g.nf[2];
+    // per wave, the window weights of the 64 records in flight (one row per
+    // lane, staged by the lane that loaded the record): a hit's weights are
+    // then read from LDS, not gathered from global memory
+    extern __shared__ unsigned char smem_raw[];
+    R* wl = reinterpret_cast<R*>(smem_raw) + int64_t(wave) * 64 * a.ws;
+    constexpr int RV = 16 / int(sizeof(R));   // elements per 16-B copy (rows are 16-B multiples)
     for (int sb = wave; sb < nsb; sb += nwaves) {
         const int o2 = (sb % ns2) * B2, o1 = ((sb / ns2) % ns1) * B1, o0 = (sb / (ns2 * ns1)) * B0;
         const int l0c = o0 + c0, l1c = o1 + c1, l2c = o2 + c2;     // lane's tile-local cell
@@ -261,6 +267,11 @@ __global__ void __launch_bounds__(kNufftThreads)
                 rec = visits[v];
                 hit = arcs_meet(rec.y, wa0, G0, B0, n0) && arcs_meet(rec.z, wa1, G1, B1, n1) &&
                       arcs_meet(rec.w, wa2, G2, B2, n2);
+                if (hit) {
+                    const uint4* src = reinterpret_cast<const uint4*>(sw + int64_t(rec.x >> 3) * a.ws);
+                    uint4* dst = reinterpret_cast<uint4*>(wl + lane * a.ws);
+                    for (int k = 0; k < a.ws / RV; ++k) dst[k] = src[k];
+                }
             }
             unsigned long long mask = __ballot(hit);
             while (mask) {
@@ -270,11 +281,16 @@ __global__ void __launch_bounds__(kNufftThreads)
                 const int p0 = __builtin_amdgcn_readlane(rec.y, i);
                 const int p1 = __builtin_amdgcn_readlane(rec.z, i);
                 const int p2 = __builtin_amdgcn_readlane(rec.w, i);
-                const int q0 = wrap_once(x0 - p0, n0), q1 = wrap_once(x1 - p1, n1),
-                          q2 = wrap_once(x2 - p2, n2);
-                const bool in = q0 < wa0 && q1 < wa1 && q2 < wa2;
+                // q = (x - p) mod n for x, p in [0, n): u = x - p + n in [1, 2n),
+                // q = min(u, u - n) unsigned (lanes past the grid get q >= n,
+                // outside the support)
+                const unsigned u0 = unsigned(x0 + (n0 - p0)), u1 = unsigned(x1 + (n1 - p1)),
+                               u2 = unsigned(x2 + (n2 - p2));
+                const unsigned q0 = min(u0, u0 - unsigned(n0)), q1 = min(u1, u1 - unsigned(n1)),
+                               q2 = min(u2, u2 - unsigned(n2));
+                const bool in = q0 < unsigned(wa0) && q1 < unsigned(wa1) && q2 < unsigned(wa2);
                 // out-of-support lanes read weight 0 of the point (in bounds)
-                const R* wp = sw + int64_t(s) * a.ws;
+                const R* wp = wl + i * a.ws;
                 const R w0 = wp[in ? q0 : 0], w1 = wp[W + (in ? q1 : 0)], w2 = wp[2 * W + (in ? q2 : 0)];
                 const V* cv = csort + int64_t(s) * a.nbatch + g0;
                 V c[CG];
