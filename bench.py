@@ -247,7 +247,7 @@ def main():
     ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-coils", type=int, default=2)
-    ap.add_argument("--phases", default="aha,type2,type1,plain,frames")
+    ap.add_argument("--phases", default="aha,type2,type1,plain,segments,frames")
     ap.add_argument("--reduce-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="reduce two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of this script "
                          "into the traffic summary profiles/pmc_traffic_<workload>.json and exit")
@@ -409,6 +409,33 @@ def main():
             e["frac_of_8TBs"] = round(ab[k] / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
         kernels_json[k] = e
 
+    # time-segmented SENSE and the multi-kernel normal operator at the same
+    # workload, L = 4 segments of a smooth synthetic rate map (interpolators
+    # fitted once, untimed); one segment costs one batched NUFFT of all coils
+    segments = None
+    if "segments" in phases:
+        from hastycucompute_amd.mri import NORMAL_IDT_T1_OP, SenseOffresonance
+        from hastycucompute_amd.offres import make_offresonance_interpolators
+        L = 4
+        zz, yy, xx = torch.meshgrid(*[torch.linspace(-1, 1, N, device=dev)] * 3, indexing="ij")
+        fmap = torch.complex(torch.full_like(xx, -10.0), 2 * math.pi * 60.0 * (xx ** 2 + 0.5 * yy - 0.3 * zz))
+        times = torch.linspace(0.0, 5e-3, samples, dtype=torch.float64, device=dev).repeat(spokes)
+        bil, cjl = make_offresonance_interpolators(fmap, times, L, 64)
+        sop = SenseOffresonance(coords, smaps, bil, cjl, tol=1e-5)
+        img1 = torch.empty_like(x)
+        fwd = run_phase(f"time-segmented forward (L={L})", lambda: sop.forward(x, out=y))
+        adj = run_phase(f"time-segmented adjoint, coil-combined (L={L})", lambda: sop.adjoint(y, combine=True, out=img1))
+        del sop
+        kernels = kernel.unsqueeze(0).expand((L,) + tuple(kernel.shape)).contiguous()
+        idt = NORMAL_IDT_T1_OP(kernels, cjl, smaps)
+        idt_t = run_phase(f"NORMAL_IDT_T1_OP (L={L})", lambda: idt.apply(x, out))
+        del idt, kernels
+        segments = {"segments": L, "forward_ms": round(fwd * 1e3, 3), "adjoint_combined_ms": round(adj * 1e3, 3),
+                    "forward_per_segment_vs_type2": round(fwd / L / res["type2"], 3) if "type2" in res else None,
+                    "adjoint_per_segment_vs_type1": round(adj / L / res["type1"], 3) if "type1" in res else None,
+                    "idt_t1_ms": round(idt_t * 1e3, 3),
+                    "idt_t1_per_kernel_vs_aha": round(idt_t / L / res["aha"], 3) if "aha" in res else None}
+
     frames = None
     if "frames" in phases:
         frames = bench_frames(dev)
@@ -444,6 +471,7 @@ def main():
             "batched_speedup": {k: round(plain[k] / res[k], 3) for k in plain if k in res},
         } if plain else None),
         "batched_frames_hipgraph": frames,
+        "time_segmented_and_multi_kernel": segments,
         "aha_pipeline_roofline": ({"achieved": round(ab["aha"] / aha_t / 1e9, 1), "peak": HBM_PEAK_GBS,
                                    "unit": "GB/s", "frac": round(ab["aha"] / aha_t / 1e9 / HBM_PEAK_GBS, 4),
                                    "algorithmic_bytes": ab["aha"]} if aha_t else None),

@@ -158,10 +158,14 @@ class _NormalT1(_NormalFamily):
                                             hfft._stream(self._device), ctypes.byref(h)))
         self._h = h
 
-    def __call__(self, x: torch.Tensor) -> torch.Tensor:
+    def apply(self, x: torch.Tensor, out: torch.Tensor) -> torch.Tensor:
         self._check_call()
         _c64(x, "x", self._n)
-        return self._apply(x, torch.empty_like(x))
+        _c64(out, "out", self._n)
+        return self._apply(x, out)
+
+    def __call__(self, x: torch.Tensor) -> torch.Tensor:
+        return self.apply(x, torch.empty_like(x))
 
 
 class _NormalT2(_NormalFamily):
