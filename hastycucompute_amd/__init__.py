@@ -7,8 +7,13 @@ The Python modules mirror the reference's corelib operator API:
 
 * ``hastycucompute_amd.fft`` -- ``nufft_plan``, ``toeplitz_kernel``,
   ``toeplitz_multiply``, ``toeplitz_multiplication`` ...
-* ``hastycucompute_amd.mri`` -- ``NORMAL_T_T1_OP``, ``SenseNufft``
+* ``hastycucompute_amd.mri`` -- ``NORMAL_T_T1_OP`` ... ``NORMAL_IDTW_T2_OP``,
+  ``SenseOffresonance`` / ``sense_forward_offresonance`` / ``sense_adjoint_offresonance``
 * ``hastycucompute_amd.op``  -- ``conjugate_gradient``, ``max_eig``
+* ``hastycucompute_amd.offres`` -- time-segmentation interpolators
+* ``hastycucompute_amd.io`` -- raw tensor files (``.htc``)
+* ``hastycucompute_amd.ops`` -- ``torch.ops.hasty.*`` registrations
+* ``hastycucompute_amd.autograd``, ``frames``, ``autotune``, ``trajectory``
 """
 from ._lib import LIB_PATH, HastyError, lib  # noqa: F401
 

@@ -198,3 +198,22 @@ def test_binary_tensor_round_trip(tmp_path):
         import_binary_tensor(p, (4, 3), torch.complex128)
     with pytest.raises(RuntimeError, match="Could not open file for reading"):
         import_binary_tensor(tmp_path / "missing.htc", (1,), torch.float32)
+
+
+def test_torch_library_ops_registered_with_fake_kernels():
+    import torch
+    from torch._subclasses.fake_tensor import FakeTensorMode
+    import hastycucompute_amd.ops  # noqa: F401
+    for name in ("nufft_type2", "nufft_type1", "toeplitz_normal", "sense_cg"):
+        assert hasattr(torch.ops.hasty, name)
+    with FakeTensorMode():
+        img = torch.empty(3, 8, 6, 4, dtype=torch.complex64)
+        k = [torch.empty(100) for _ in range(3)]
+        assert torch.ops.hasty.nufft_type2(img, k, 1e-5).shape == (3, 100)
+        kd = torch.empty(3, 100, dtype=torch.complex64)
+        assert torch.ops.hasty.nufft_type1(kd, k, [8, 6, 4], 1e-5).shape == (3, 8, 6, 4)
+        x = torch.empty(8, 6, 4, dtype=torch.complex64)
+        kern = torch.empty(16, 12, 8, dtype=torch.complex64)
+        S = torch.empty(2, 8, 6, 4, dtype=torch.complex64)
+        assert torch.ops.hasty.toeplitz_normal(x, kern, S).shape == x.shape
+        assert torch.ops.hasty.sense_cg(x, kern, S, 5).shape == x.shape
