@@ -140,7 +140,13 @@ def reduce_pmc(fetch_dir, write_dir, workload):
     """Average FETCH_SIZE / WRITE_SIZE (KiB per dispatch) of each measured
     kernel, scaled by the factor that maps the calibration kernel's reading to
     its known 1 GiB, into bytes per launch."""
-    out = {"workload": workload, "counters_kib_per_dispatch": {}, "calibration": {}, "kernels": {}}
+    cmd = ("cd /tmp && export TMPDIR=/tmp && cd $REPO && rocprofv3 --pmc {c} --output-format csv "
+           "-d {d} -o run -- python -u bench.py --steps 1 --warmup 0 --no-cpu-baseline "
+           "--phases aha,type2,type1")
+    out = {"workload": workload,
+           "commands": [cmd.format(c="FETCH_SIZE", d=fetch_dir), cmd.format(c="WRITE_SIZE", d=write_dir),
+                        f"python bench.py --reduce-pmc {fetch_dir} {write_dir}"],
+           "counters_kib_per_dispatch": {}, "calibration": {}, "kernels": {}}
     scaled = {}
     for counter, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
         rows = _pmc_rows(d)
