@@ -182,6 +182,12 @@ template <class V> struct PassArgs {
     // a wave-uniform 64-bit base plus a 32-bit per-lane offset
     uint32_t in_ib, in_sb, out_ib, out_sb, psf_ib, psf_sb;
     uint32_t lm1_rb, lm2_rb, sm1_rb, sm2_rb, sio_rb;   // diagonal row strides in bytes
+    // quarter turns folded into the step-1 twiddles (set by the host with
+    // the half-line specs for a line shifted by -L/4, the centred modes of
+    // the NUFFT): twist_out multiplies output k by i^(k mod 4) relative to
+    // the unshifted transform, twist_in input p by i^(p mod 4), each taken
+    // through the pass's sign convention (see the twiddle fill)
+    int twist_out, twist_in;
 };
 
 template <int L1, int L2> struct RegShape {
@@ -205,13 +211,28 @@ enum PassMode : int {
     PM_ROW_STORE = 3,  // contiguous lines, diagonals / deapodisation / sums on store
 };
 
-// Line shapes known at compile time (the Toeplitz passes): the input is the
-// first half of the line (in_shift 0, in_len L/2), or the output keeps
-// positions [L/2 - 1, L - 1) (out_shift L/2 - 1, out_len L/2).  Positions
-// that are zero or dropped then cost nothing: no load, no arithmetic on the
-// zeros (the passes are built with -fno-signed-zeros so x + 0 folds), no
-// DFT output that is not stored.
-enum PassSpec : int { PS_NONE = 0, PS_HALF_IN = 1, PS_HALF_OUT = 2 };
+// Line shapes known at compile time: the input is the first half of the
+// line (in_shift 0, in_len L/2), or the output keeps positions
+// [L/2 - 1, L - 1) (out_shift L/2 - 1, out_len L/2; the Toeplitz crop) or
+// [0, L/2) (HALF_OUT0).  Positions that are zero or dropped then cost
+// nothing: no load, no arithmetic on the zeros (the passes are built with
+// -fno-signed-zeros so x + 0 folds), no DFT output that is not stored.
+// A line whose half sits at shift -L/4 (the NUFFT's centred modes, wrapped
+// around 0) runs as HALF_IN / HALF_OUT0 with a quarter-turn twist: a shift
+// by s multiplies the other side of the transform by W^(k s), and for
+// s = -L/4 that is a power of i fixed by k mod 4.
+enum PassSpec : int { PS_NONE = 0, PS_HALF_IN = 1, PS_HALF_OUT = 2, PS_HALF_OUT0 = 4 };
+
+// v * i^r
+template <class V>
+__device__ inline V quarter_turn(V v, int r) {
+    switch (r & 3) {
+        case 1: return cmake<V>(-v.y, v.x);
+        case 2: return cmake<V>(-v.x, -v.y);
+        case 3: return cmake<V>(v.y, -v.x);
+        default: return v;
+    }
+}
 
 // a * b (sgn = 1) or a * conj(b) (sgn = -1); sgn is wave-uniform
 template <class V>
@@ -271,6 +292,8 @@ __global__ void __launch_bounds__(512) fft2_pass_kernel(PassArgs<V> a) {
     constexpr bool PSF = MODE == PM_COL_PSF;
     constexpr bool HALF_IN = (SPEC & PS_HALF_IN) != 0;
     constexpr bool HALF_OUT = (SPEC & PS_HALF_OUT) != 0;
+    constexpr bool HALF_OUT0 = (SPEC & PS_HALF_OUT0) != 0;
+    constexpr bool HALF_KEEP = HALF_OUT || HALF_OUT0;   // L/2 outputs kept
     constexpr uint32_t VB = sizeof(V);
     // column passes stream their lines (nontemporal); the row load pass
     // fetches the next batch element's input while transforming the current
@@ -304,8 +327,16 @@ __global__ void __launch_bounds__(512) fft2_pass_kernel(PassArgs<V> a) {
     //   twb[k1 * L2 + m]  = W_L^{k1 m}   (inverse step 1 of the PSF pass, k1 < L1)
     V* twa = lds + int64_t(TX) * a.ldp;
     V* twb = twa + L;
+    // With a twist, entry (p2, k1) also carries i^r, r = k1 (output twist:
+    // step 2 keeps k1 = k mod 4 when L1 % 4 == 0) plus p2 (input twist:
+    // p2 = p mod 4 when L2 % 4 == 0).  The table is conjugated on use when
+    // SIGN > 0, and the twist conjugates with it: both directions take i^r.
+    // (only the half-line kernels take twists)
     for (int k = tid; k < L; k += blockDim.x) {
-        twa[k] = a.tw[((k / L1) * (k % L1)) % L];
+        int r = 0;
+        if constexpr (HALF_IN && !PSF) r += a.twist_out ? k % L1 : 0;
+        if constexpr (HALF_OUT0) r += a.twist_in ? k / L1 : 0;
+        twa[k] = quarter_turn(a.tw[((k / L1) * (k % L1)) % L], r);
         if (PSF) twb[k] = a.tw[((k / L2) * (k % L2)) % L];
     }
     __syncthreads();
@@ -440,6 +471,11 @@ __global__ void __launch_bounds__(512) fft2_pass_kernel(PassArgs<V> a) {
             }
             RegDFT<L1, SIGN, V>::run(v);
             const V* tw = twa + t * L1;
+            // an input twist is the same for every p1 of this thread, so it
+            // applies after the L1-point DFT, to k1 = 0 as well
+            if constexpr (HALF_OUT0) {
+                if (a.twist_in) v[0] = twiddle_mul<V, SIGN>(v[0], tw, 0);
+            }
 #pragma unroll
             for (int k1 = 1; k1 < L1; ++k1) v[k1] = twiddle_mul<V, SIGN>(v[k1], tw, k1);
 #pragma unroll
@@ -486,15 +522,16 @@ __global__ void __launch_bounds__(512) fft2_pass_kernel(PassArgs<V> a) {
             __syncthreads();
             if (line_ok) {
                 // outputs j = t + T m: HALF_OUT keeps j < L/2 at line positions
-                // q = j + L/2 - 1; otherwise j < out_len at q = j + out_shift mod L,
-                // taken in rounds of NACC outputs per thread (the coil sum has
-                // one round: the host keeps out_len <= NACC T)
-                constexpr int NR = HALF_OUT ? 1 : (L + T * NACC - 1) / (T * NACC);
+                // q = j + L/2 - 1, HALF_OUT0 at q = j; otherwise j < out_len at
+                // q = j + out_shift mod L, taken in rounds of NACC outputs per
+                // thread (the coil sum has one round: the host keeps
+                // out_len <= NACC T)
+                constexpr int NR = HALF_KEEP ? 1 : (L + T * NACC - 1) / (T * NACC);
                 char* dst = reinterpret_cast<char*>(a.out + (sum_b ? 0 : b * a.out_b) + int64_t(o) * a.out_o);
                 const uint32_t lo = uint32_t(icl) * a.out_ib;
 #pragma unroll
                 for (int r = 0; r < NR; ++r) {
-                    if (!HALF_OUT && T * NACC * r >= out_len) break;
+                    if (!HALF_KEEP && T * NACC * r >= out_len) break;
                     int jj[NACC];
                     bool ok[NACC];
                     V val[NACC];
@@ -502,9 +539,9 @@ __global__ void __launch_bounds__(512) fft2_pass_kernel(PassArgs<V> a) {
                     for (int m = 0; m < NACC; ++m) {
                         const int j = t + T * (NACC * r + m);
                         int q;
-                        if constexpr (HALF_OUT) {
+                        if constexpr (HALF_KEEP) {
                             ok[m] = T * (m + 1) <= L / 2 || j < L / 2;
-                            q = j + L / 2 - 1;
+                            q = HALF_OUT ? j + L / 2 - 1 : j;
                         } else {
                             ok[m] = j < out_len;
                             q = j + out_shift;
@@ -547,9 +584,9 @@ __global__ void __launch_bounds__(512) fft2_pass_kernel(PassArgs<V> a) {
                     const int q = t + OSTRIDE * k;
                     int j;
                     bool ok;
-                    if constexpr (HALF_OUT) {
-                        // kept: L/2 - 1 <= q < L - 1, q in [OSTRIDE k, OSTRIDE (k + 1))
-                        constexpr int lo_q = L / 2 - 1, hi_q = L - 1;
+                    if constexpr (HALF_KEEP) {
+                        // kept: lo_q <= q < hi_q, q in [OSTRIDE k, OSTRIDE (k + 1))
+                        constexpr int lo_q = HALF_OUT ? L / 2 - 1 : 0, hi_q = HALF_OUT ? L - 1 : L / 2;
                         j = q - lo_q;
                         const bool none = OSTRIDE * (k + 1) <= lo_q || OSTRIDE * k >= hi_q;
                         const bool all = OSTRIDE * k >= lo_q && OSTRIDE * (k + 1) <= hi_q;
@@ -567,7 +604,7 @@ __global__ void __launch_bounds__(512) fft2_pass_kernel(PassArgs<V> a) {
     }
     if constexpr (MODE == PM_ROW_STORE) {
         if (sum_b && line_ok) {
-            const int out_len = HALF_OUT ? L / 2 : a.out_len;
+            const int out_len = HALF_KEEP ? L / 2 : a.out_len;
             if (a.groups > 1) {
                 // this group's partial sum, rows of out_len values
                 V* part = a.partial + (bblk * a.ni + i) * int64_t(out_len);

@@ -50,6 +50,7 @@ void launch_mode(const PassArgs<V>& a, int mode, int spec, const char* tag, hipS
         case PM_COL:
             if (SIGN == F && spec == PS_HALF_IN) return launch_one<V, L1, L2, SIGN, PM_COL, PS_HALF_IN>(a, tag, s);
             if (SIGN == I && spec == PS_HALF_OUT) return launch_one<V, L1, L2, SIGN, PM_COL, PS_HALF_OUT>(a, tag, s);
+            if (SIGN == I && spec == PS_HALF_OUT0) return launch_one<V, L1, L2, SIGN, PM_COL, PS_HALF_OUT0>(a, tag, s);
             return launch_one<V, L1, L2, SIGN, PM_COL, PS_NONE>(a, tag, s);
         case PM_COL_PSF:
             if (SIGN == F && spec == (PS_HALF_IN | PS_HALF_OUT))
@@ -60,7 +61,21 @@ void launch_mode(const PassArgs<V>& a, int mode, int spec, const char* tag, hipS
             return launch_one<V, L1, L2, SIGN, PM_ROW_LOAD, PS_NONE>(a, tag, s);
         default:
             if (SIGN == I && spec == PS_HALF_OUT) return launch_one<V, L1, L2, SIGN, PM_ROW_STORE, PS_HALF_OUT>(a, tag, s);
+            if (SIGN == I && spec == PS_HALF_OUT0) return launch_one<V, L1, L2, SIGN, PM_ROW_STORE, PS_HALF_OUT0>(a, tag, s);
             return launch_one<V, L1, L2, SIGN, PM_ROW_STORE, PS_NONE>(a, tag, s);
+    }
+}
+
+// Does launch_mode have a kernel for this (mode, sign, spec)?  Otherwise the
+// general kernel runs, which must then get neither the spec nor a twist.
+inline bool spec_built(int mode, int sign, int spec) {
+    if (spec == PS_NONE) return true;
+    switch (mode) {
+        case PM_COL:
+            return (sign < 0 && spec == PS_HALF_IN) || (sign > 0 && (spec == PS_HALF_OUT || spec == PS_HALF_OUT0));
+        case PM_COL_PSF: return sign < 0 && spec == (PS_HALF_IN | PS_HALF_OUT);
+        case PM_ROW_LOAD: return sign < 0 && spec == PS_HALF_IN;
+        default: return sign > 0 && (spec == PS_HALF_OUT || spec == PS_HALF_OUT0);
     }
 }
 
@@ -100,8 +115,26 @@ bool launch_shape(PassArgs<V> a, int sign, int mode, const char* tag, hipStream_
     a.sm2_rb = uint32_t(a.sm2.sr * int64_t(vb));
     a.sio_rb = uint32_t(a.sio.sr * int64_t(vb));
     int spec = PS_NONE;
-    if (L % 2 == 0 && a.in_shift == 0 && a.in_len == L / 2) spec |= PS_HALF_IN;
-    if (L % 2 == 0 && a.out_shift == L / 2 - 1 && a.out_len == L / 2) spec |= PS_HALF_OUT;
+    a.twist_in = a.twist_out = 0;
+    // a half line at shift -L/4 (positions [3L/4, L) and [0, L/4)) runs as the
+    // half line at shift 0 with a quarter-turn twist on the other side
+    const int64_t in_sh = ((int64_t(a.in_shift) % L) + L) % L, out_sh = ((int64_t(a.out_shift) % L) + L) % L;
+    if (L % 2 == 0 && a.in_shift == 0 && a.in_len == L / 2) {
+        spec |= PS_HALF_IN;
+    } else if (L % 4 == 0 && L1 % 4 == 0 && a.in_len == L / 2 && in_sh == 3 * L / 4) {
+        spec |= PS_HALF_IN;
+        a.twist_out = 1;
+    }
+    if (L % 2 == 0 && a.out_shift == L / 2 - 1 && a.out_len == L / 2) {
+        spec |= PS_HALF_OUT;
+    } else if (L % 4 == 0 && L2 % 4 == 0 && a.out_len == L / 2 && out_sh == 3 * L / 4) {
+        spec |= PS_HALF_OUT0;
+        a.twist_in = 1;
+    }
+    if (!spec_built(mode, sign, spec) || (a.twist_in && a.twist_out)) {
+        spec = PS_NONE;
+        a.twist_in = a.twist_out = 0;
+    }
     // HASTY_COL_TX / HASTY_PSF_TX: lines per block of the column passes
     // (tuning; defaults measured at 128^3 x 32 coils: 32 columns = 256
     // contiguous bytes of c64 per row for the plain column passes, 16 for
