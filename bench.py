@@ -115,7 +115,7 @@ def cpu_baseline(N, C, smaps_np, x_np, kernel_np, sample_coils):
 PMC_KERNELS = {
     "toep_P3_z_psf": "fft2_pass_kernel<HIP_vector_type<float, 2u>, 16, 16, -1, 1, 3>",
     "nufft_spread": "spread_kernel<HIP_vector_type<float, 2u>, 32>",
-    "nufft_interp": "interp_kernel<HIP_vector_type<float, 2u>, 6,",
+    "nufft_interp": "interp_pair_kernel<6, 16, 16, 16>",
 }
 CALIB_SUBSTR = "neg_kernel"
 CALIB_BYTES = 1 << 30

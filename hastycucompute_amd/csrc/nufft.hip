@@ -469,6 +469,18 @@ NufftPlan* nufft_makeplan(int type, int dim, const int64_t* nmodes, int sign, in
     p->max_batch = (opts && opts->max_batch > 0) ? std::min(opts->max_batch, ntransf) : ntransf;
     p->max_sub = (opts && opts->max_subproblem > 0) ? opts->max_subproblem : (type == 2 ? 8192 : 2048);
     if (!p->gm && type == 2) set_lds_limit(interp_ptr(p.get()), interp_lds_bytes(p.get(), p->group));
+    // complex64 plans with a batch and the standard tile interpolate two
+    // transforms per block (16-B LDS reads), when the pair's tile fits in LDS
+    // and no transform group was asked for (HASTY_INTERP_PAIR=0 turns it off)
+    if (!p->gm && type == 2 && !f64 && ntransf >= 2 && !(opts && opts->coil_group > 0)) {
+        static const char* env = std::getenv("HASTY_INTERP_PAIR");
+        const bool std3 = dim == 3 && p->T[0] == 16 && p->T[1] == 16 && p->T[2] == 16;
+        void* fn = std3 ? nufft_interp_pair_ptr(dim, p->w) : nullptr;
+        if (fn && interp_lds_bytes(p.get(), 2) <= kLdsMax && !(env && env[0] == '0')) {
+            p->pair = true;
+            set_lds_limit(fn, interp_lds_bytes(p.get(), 2));
+        }
+    }
     // deapodisation factors 1/Phi(2 pi m / nf) per slot, centred modes
     for (int s = 0; s < 3; ++s) {
         const int64_t n = p->n[s];
@@ -789,9 +801,18 @@ void execute_impl(NufftPlan* p, V* c, V* f, const NufftHooks& hk, int ntransf) {
                 ga.xchunk = xcd_chunk(false);
                 HASTY_PROF("nufft_interp", s);
                 // XCD-aware numbering (xcd_block) over a 1-D grid
-                const int64_t nblocks = xcd_grid(p->nsub_interp, ngroups, ga.xchunk);
-                hipLaunchKernelGGL(reinterpret_cast<GridFn<V>>(interp_ptr(p)), dim3(unsigned(nblocks)),
-                                   dim3(kNufftThreads), interp_lds_bytes(p, ga.group), s, ga);
+                if (p->pair && nb >= 2) {
+                    ga.group = 2;
+                    ga.ngroups = int(ceil_div(nb, 2));
+                    const int64_t nblocks = xcd_grid(p->nsub_interp, ga.ngroups, ga.xchunk);
+                    hipLaunchKernelGGL(reinterpret_cast<GridFn<V>>(nufft_interp_pair_ptr(p->dim, p->w)),
+                                       dim3(unsigned(nblocks)), dim3(kInterpPairThreads),
+                                       interp_lds_bytes(p, 2), s, ga);
+                } else {
+                    const int64_t nblocks = xcd_grid(p->nsub_interp, ngroups, ga.xchunk);
+                    hipLaunchKernelGGL(reinterpret_cast<GridFn<V>>(interp_ptr(p)), dim3(unsigned(nblocks)),
+                                       dim3(kNufftThreads), interp_lds_bytes(p, ga.group), s, ga);
+                }
             }
             HASTY_CHECK_HIP(hipGetLastError());
         } else {

@@ -52,6 +52,7 @@ struct NufftPlan {
     int64_t M = 0;
     bool have_pts = false;
     bool gm = false;             // global-memory spread / interp kernels
+    bool pair = false;           // type 2: paired interpolation (two transforms per block)
     int* sidx = nullptr;
     int* sinv = nullptr;
     int* sl0 = nullptr;          // [3][M]

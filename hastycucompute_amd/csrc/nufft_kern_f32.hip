@@ -30,6 +30,21 @@ template <> void* nufft_spread_ptr<float2>(int cg) {
     throw Error(HASTY_ERR_INTERNAL, "no spreading kernel for this transform group");
 }
 
+void* nufft_interp_pair_ptr(int dim, int w) {
+    // 3-D only: a 2-D window is one z plane, whose 16-B reads the compiler
+    // hoists together past the register budget
+    if (dim == 3) {
+        switch (w) {
+            case 2: return reinterpret_cast<void*>(&interp_pair_kernel<2, 16, 16, 16>);
+            case 3: return reinterpret_cast<void*>(&interp_pair_kernel<3, 16, 16, 16>);
+            case 4: return reinterpret_cast<void*>(&interp_pair_kernel<4, 16, 16, 16>);
+            case 5: return reinterpret_cast<void*>(&interp_pair_kernel<5, 16, 16, 16>);
+            case 6: return reinterpret_cast<void*>(&interp_pair_kernel<6, 16, 16, 16>);
+        }
+    }
+    return nullptr;
+}
+
 template <> void* nufft_interp_fixed_ptr<float2>(int dim, int w) {
     if (dim == 3) {
         switch (w) {

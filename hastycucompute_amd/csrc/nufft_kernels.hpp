@@ -185,6 +185,109 @@ __global__ void __launch_bounds__(kNufftThreads) interp_kernel(GridArgs<V> a) {
 }
 
 // ---------------------------------------------------------------------------
+// Paired interpolation (complex64, compile-time tile): one block per (tile
+// subproblem, pair of transforms).  The pair is staged interleaved, one 16-B
+// value (transform 2k, transform 2k + 1) per padded cell, so every cell of a
+// point's window is a single aligned 16-B LDS read serving both transforms:
+// on gfx950 16-B LDS reads move about twice the bytes per clock of the 8-B
+// pairs the single-transform kernel issues.  The tile of a pair takes twice
+// the LDS (one block per CU at 16^3, w = 6), so the block has 1024 threads to
+// keep 16 waves per CU.  An odd last transform stages zeros in its partner
+// slot and stores nothing for it.
+// ---------------------------------------------------------------------------
+constexpr int kInterpPairThreads = 1024;
+
+template <int W, int TZ, int TY, int TX>
+__global__ void __launch_bounds__(kInterpPairThreads) interp_pair_kernel(GridArgs<float2> a) {
+    extern __shared__ unsigned char smem_raw[];
+    float4* lds = reinterpret_cast<float4*>(smem_raw);
+    int isub, igroup;
+    if (!xcd_block(a.nsub, a.ngroups, a.xchunk, isub, igroup)) return;
+    const int4 sub = a.subs[isub];
+    int t0[3];
+    tile_origin(a.g, sub.x, t0);
+    constexpr int wz = TZ == 1 ? 1 : W;
+    constexpr int Pz = TZ == 1 ? 1 : TZ + W - 1, Py = TY + W - 1, Px = TX + W - 1;
+    constexpr int cells = Pz * Py * Px;
+    const int g0 = 2 * igroup;
+    const bool two = g0 + 1 < a.nbatch;
+    const int64_t G = a.g.nf[0] * a.g.nf[1] * a.g.nf[2];
+    const float2* f0 = a.fine + int64_t(g0) * G;
+    const float2* f1 = two ? f0 + G : f0;
+    // stage, 4 cells (8 loads) in flight per thread
+    for (int base = threadIdx.x; base < cells; base += 4 * blockDim.x) {
+        float2 v0[4], v1[4];
+        int off[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int c = base + u * blockDim.x;
+            off[u] = -1;
+            if (c < cells) {
+                const int px = c % Px;
+                const int py = (c / Px) % Py;
+                const int pz = c / (Px * Py);
+                int gz = t0[0] + pz; gz -= gz >= a.g.nf[0] ? int(a.g.nf[0]) : 0;
+                int gy = t0[1] + py; gy -= gy >= a.g.nf[1] ? int(a.g.nf[1]) : 0;
+                int gx = t0[2] + px; gx -= gx >= a.g.nf[2] ? int(a.g.nf[2]) : 0;
+                off[u] = int((int64_t(gz) * a.g.nf[1] + gy) * a.g.nf[2] + gx);
+                v0[u] = f0[off[u]];
+                v1[u] = two ? f1[off[u]] : make_float2(0.f, 0.f);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (off[u] >= 0) lds[base + u * blockDim.x] = make_float4(v0[u].x, v0[u].y, v1[u].x, v1[u].y);
+    }
+    __syncthreads();
+    for (int s = sub.y + threadIdx.x; s < sub.y + sub.z; s += blockDim.x) {
+        const int lz = a.sl0[s] - t0[0];
+        const int ly = a.sl0[a.M + s] - t0[1];
+        const int lx = a.sl0[2 * a.M + s] - t0[2];
+        const float* w = a.sw + int64_t(s) * a.ws;
+        float wz_[W], wy[W], wx[W];
+#pragma unroll
+        for (int q = 0; q < W; ++q) {
+            wz_[q] = w[q];
+            wy[q] = w[W + q];
+            wx[q] = w[2 * W + q];
+        }
+        const int orig = a.sidx[s];
+        const float4* tb = lds + (lz * Py + ly) * Px + lx;
+        float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+        // one z plane of the window per iteration: unrolled whole, its 16-B
+        // reads (4 VGPRs each) would be hoisted together and spill
+#pragma unroll 1
+        for (int qz = 0; qz < wz; ++qz) {
+            float4 accy = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int qy = 0; qy < W; ++qy) {
+                float4 row[W];
+#pragma unroll
+                for (int qx = 0; qx < W; ++qx) row[qx] = tb[(qz * Py + qy) * Px + qx];
+                float4 accx = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+                for (int qx = 0; qx < W; ++qx) {
+                    accx.x += wx[qx] * row[qx].x;
+                    accx.y += wx[qx] * row[qx].y;
+                    accx.z += wx[qx] * row[qx].z;
+                    accx.w += wx[qx] * row[qx].w;
+                }
+                accy.x += wy[qy] * accx.x;
+                accy.y += wy[qy] * accx.y;
+                accy.z += wy[qy] * accx.z;
+                accy.w += wy[qy] * accx.w;
+            }
+            acc.x += wz_[qz] * accy.x;
+            acc.y += wz_[qz] * accy.y;
+            acc.z += wz_[qz] * accy.z;
+            acc.w += wz_[qz] * accy.w;
+        }
+        store_point(a, int64_t(g0), orig, make_float2(acc.x, acc.y));
+        if (two) store_point(a, int64_t(g0 + 1), orig, make_float2(acc.z, acc.w));
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Spreading, formulated as a gather: one block per (tile subproblem, group of
 // CG transforms).  Each wave owns a sub-block of 64 cells of the tile
 // (4 x 4 x 4 in 3-D, 1 x 8 x 8 in 2-D) and each lane one cell of it, with
@@ -591,6 +694,8 @@ template <class V> void* nufft_kernel_ptr(int which, int w);
 // interpolation kernel with the tile fixed at compile time (16^3 in 3-D,
 // 32^2 in 2-D), or nullptr when this width has none
 template <class V> void* nufft_interp_fixed_ptr(int dim, int w);
+// paired interpolation kernel (complex64, standard 3-D tile, w <= 6), or NULL
+void* nufft_interp_pair_ptr(int dim, int w);
 // spreading kernel for CG transforms per block (1, 2, 4, 8, 16, 32 / 16 for complex128)
 template <class V> void* nufft_spread_ptr(int cg);
 
