@@ -214,13 +214,16 @@ __global__ void __launch_bounds__(kInterpPairThreads) interp_pair_kernel(GridArg
     const int64_t G = a.g.nf[0] * a.g.nf[1] * a.g.nf[2];
     const float2* f0 = a.fine + int64_t(g0) * G;
     const float2* f1 = two ? f0 + G : f0;
-    // stage, 4 cells (8 loads) in flight per thread
-    for (int base = threadIdx.x; base < cells; base += 4 * blockDim.x) {
-        float2 v0[4], v1[4];
-        int off[4];
+    // stage: every load of the tile in flight at once (one memory latency per
+    // block, which has the CU to itself while it stages)
+    constexpr int SU = (cells + kInterpPairThreads - 1) / kInterpPairThreads;
+    {
+        const int base = int(threadIdx.x);
+        float2 v0[SU], v1[SU];
+        int off[SU];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int c = base + u * blockDim.x;
+        for (int u = 0; u < SU; ++u) {
+            const int c = base + u * kInterpPairThreads;
             off[u] = -1;
             if (c < cells) {
                 const int px = c % Px;
@@ -235,8 +238,8 @@ __global__ void __launch_bounds__(kInterpPairThreads) interp_pair_kernel(GridArg
             }
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (off[u] >= 0) lds[base + u * blockDim.x] = make_float4(v0[u].x, v0[u].y, v1[u].x, v1[u].y);
+        for (int u = 0; u < SU; ++u)
+            if (off[u] >= 0) lds[base + u * kInterpPairThreads] = make_float4(v0[u].x, v0[u].y, v1[u].x, v1[u].y);
     }
     __syncthreads();
     for (int s = sub.y + threadIdx.x; s < sub.y + sub.z; s += blockDim.x) {
