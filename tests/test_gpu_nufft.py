@@ -167,6 +167,52 @@ def test_fp64_plan():
     report("fp64 type2 tol 1e-10", rel_l2(host(o2)[0], ndft_type2(f[0], coords)), 1e-9)
 
 
+@pytest.mark.parametrize("kw", [{}, {"max_subproblem": 200}])
+def test_paired_interpolation_is_bitwise_single(kw):
+    """Type 2 on the 16^3 tile interpolates two transforms per block from
+    interleaved LDS cells, after setpts reorders each subproblem's points by
+    LDS bank class.  Each point's value is the same sum in the same order as
+    in the one-transform kernel (an explicit coil_group keeps that kernel and
+    the cell order), so the outputs agree bit for bit; odd batch (the last
+    pair stages a zero partner), and split subproblems with a small cap."""
+    from hastycucompute_amd import fft as hf
+    from hastycucompute_amd import synth
+    shape, T = (24, 24, 24), 5
+    coords = synth.kooshball(300, 48)
+    M = len(coords[0])
+    rng = np.random.default_rng(31)
+    f = crandn(rng, (T,) + shape)
+    outs = []
+    for opts in (dict(kw), dict(kw, coil_group=1)):
+        p2 = hf.nufft_plan.make(hf.nufft_opts(nmodes=shape, ntransf=T, tol=EPS, **opts), hf.nufft_type.TYPE_2)
+        p2.setpts([cuda(c) for c in coords])
+        out = torch.empty((T, M), dtype=torch.complex64, device="cuda")
+        p2.execute(cuda(f), out)
+        outs.append(host(out))
+    assert np.array_equal(outs[0], outs[1])
+    sel = rng.choice(M, 1000, replace=False)
+    sub = [x[sel] for x in coords]
+    for t in (0, T - 1):
+        report(f"paired type2 {kw} t={t} vs NDFT (subset)", rel_l2(outs[0][t][sel], ndft_type2(f[t], sub)), TOL)
+
+
+def test_fp64_plan_standard_tile():
+    """A complex128 type-2 plan at eps 1e-5 on the 16^3 tile (its 16-B cells
+    take the bank-class point order) against the FP64 NDFT."""
+    from hastycucompute_amd import fft as hf
+    rng = np.random.default_rng(13)
+    shape, M, T = (24, 24, 24), 3000, 2
+    coords = [x.astype(np.float64) for x in rand_coords(rng, 3, M)]
+    p2 = hf.nufft_plan.make(hf.nufft_opts(nmodes=shape, ntransf=T, tol=EPS), hf.nufft_type.TYPE_2, torch.float64)
+    p2.setpts([cuda(x) for x in coords])
+    f = crandn(rng, (T,) + shape, np.complex128)
+    out = torch.empty((T, M), dtype=torch.complex128, device="cuda")
+    p2.execute(cuda(f), out)
+    g = host(out)
+    for t in range(T):
+        report(f"fp64 type2 16^3 tile t={t}", rel_l2(g[t], ndft_type2(f[t], coords)), TOL)
+
+
 def test_plan_misuse_raises():
     from hastycucompute_amd import fft as hf
     p = hf.nufft_plan.make(hf.nufft_opts(nmodes=(8, 8, 8)), hf.nufft_type.TYPE_2)
