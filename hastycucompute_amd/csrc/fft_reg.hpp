@@ -412,6 +412,20 @@ __global__ void __launch_bounds__(512) fft2_pass_kernel(PassArgs<V> a) {
         pre(a.lm1, a.lm1_rb, lw1);
         pre(a.lm2, a.lm2_rb, lw2);
     }
+    // deapodisation of the prefetching row pass, zy * dx[j] per held input,
+    // read once per block (the same for every batch element)
+    R dxw[PREFETCH ? L1 : 1];
+    if constexpr (PREFETCH) {
+        if (a.dx) {
+#pragma unroll
+            for (int p1 = 0; p1 < L1; ++p1) {
+                int j;
+                bool ok;
+                in_pos(p1, 0, L / 2, j, ok);
+                dxw[p1] = ok ? zy * a.dx[j] : R(0);
+            }
+        }
+    }
     // coil-sum accumulators: outputs j = t + T m, m < NACC (the coil sum is
     // used with a crop to at most L/2 outputs; the host checks out_len)
     constexpr int NACC = (L / 2 + T - 1) / T;
@@ -464,9 +478,15 @@ __global__ void __launch_bounds__(512) fft2_pass_kernel(PassArgs<V> a) {
                 use(a.lm1, a.lm1_rb, lw1);
                 use(a.lm2, a.lm2_rb, lw2);
                 if (a.dx) {
+                    if constexpr (PREFETCH) {
 #pragma unroll
-                    for (int p1 = 0; p1 < L1; ++p1)
-                        if (okin[p1]) v[p1] = cscale(v[p1], zy * a.dx[jin[p1]]);
+                        for (int p1 = 0; p1 < L1; ++p1)
+                            if (okin[p1]) v[p1] = cscale(v[p1], dxw[p1]);
+                    } else {
+#pragma unroll
+                        for (int p1 = 0; p1 < L1; ++p1)
+                            if (okin[p1]) v[p1] = cscale(v[p1], zy * a.dx[jin[p1]]);
+                    }
                 }
             }
             RegDFT<L1, SIGN, V>::run(v);
