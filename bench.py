@@ -114,7 +114,7 @@ def cpu_baseline(N, C, smaps_np, x_np, kernel_np, sample_coils):
 # calibration kernel: torch.neg over 1 GiB reads and writes exactly 2^30 B.
 PMC_KERNELS = {
     "toep_P3_z_psf": "fft2_pass_kernel<HIP_vector_type<float, 2u>, 16, 16, -1, 1, 3>",
-    "nufft_spread": "spread_kernel<HIP_vector_type<float, 2u>, 32>",
+    "nufft_spread": "spread_kernel_w6<HIP_vector_type<float, 2u>, 32>",
     "nufft_interp": "interp_pair_kernel<6, 16, 16, 16>",
 }
 CALIB_SUBSTR = "neg_kernel"
@@ -170,6 +170,129 @@ def reduce_pmc(fetch_dir, write_dir, workload):
     with open(path, "w") as fh:
         json.dump(out, fh, indent=1)
     print(json.dumps(out, indent=1))
+
+
+_TRACE_CMD = ("cd /tmp && export TMPDIR=/tmp && cd $REPO && rocprofv3 --kernel-trace --stats --output-format csv "
+              "-d {d} -o run -- python -u bench.py --steps 3 --warmup 1 --no-cpu-baseline")
+# the launches this script times after its calibration kernel: (apply, type 2,
+# type 1) three times, each as (label, kernel-name substring)
+_TIMED_SEQ = ([("toep_P1_x_pad_fft", "<c64, 16, 16, -1, 2, 1>"), ("toep_P2_y_pad_fft", "<c64, 16, 16, -1, 0, 1>"),
+               ("toep_P3_z_psf", "<c64, 16, 16, -1, 1, 3>"), ("toep_P4_y_ifft_crop", "<c64, 16, 16, 1, 0, 2>"),
+               ("toep_P5_x_ifft_crop", "<c64, 16, 16, 1, 3, 2>"), ("toep_P5 sum_partials", "sum_partials")] +
+              [("fwd_x_pad_fft", "<c64, 16, 16, -1, 2, 1>"), ("fwd_y_pad_fft", "<c64, 16, 16, -1, 0, 1>"),
+               ("fwd_z_pad_fft", "<c64, 16, 16, -1, 0, 1>"), ("nufft_interp", "interp_pair_kernel<6,")] +
+              [("nufft_sort_values", "sort_values_kernel<c64"), ("nufft_spread", "spread_kernel_w6<c64, 32>"),
+               ("nufft_spread_reduce", "reduce_split_tiles"), ("inv_z_fft_crop", "<c64, 16, 16, 1, 0, 4>"),
+               ("inv_y_fft_crop", "<c64, 16, 16, 1, 0, 4>"), ("inv_x_fft_crop", "<c64, 16, 16, 1, 3, 4>")]) * 3
+
+
+def _short_kernel(name):
+    import re
+    name = name.replace("HIP_vector_type<float, 2u>", "c64").replace("HIP_vector_type<double, 2u>", "c128")
+    name = re.sub(r"\(hasty::.*$", "", name)
+    name = re.sub(r"\(.*$", "", name)
+    return name.replace("void ", "")[:100]
+
+
+def summarize_trace(trace_dir, workload, top=30):
+    """Markdown summary of a rocprofv3 --kernel-trace run of this script:
+    launches grouped by kernel and launch shape, then the timed launches."""
+    import collections
+    import csv
+    import glob
+    files = glob.glob(os.path.join(trace_dir, "**", "*kernel_trace*.csv"), recursive=True)
+    if not files:
+        raise SystemExit(f"no kernel_trace CSV under {trace_dir}")
+    rows = [r for f in files for r in csv.DictReader(open(f))]
+    groups = collections.defaultdict(list)
+    total = 0.0
+    for r in rows:
+        d = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6
+        total += d
+        key = (_short_kernel(r["Kernel_Name"]), f'{r["Grid_Size_X"]}x{r["Grid_Size_Y"]}', r["Workgroup_Size_X"],
+               r["VGPR_Count"], r["LDS_Block_Size"])
+        groups[key].append(d)
+    N, C, spokes, samples = WORKLOADS[workload]
+    out = [f"# Kernel trace: {workload} ({N}^3 image, {C} coils, {spokes * samples / 1e6:.1f}M kooshball points "
+           f"per coil, {2 * N}^3 Toeplitz grid)", "",
+           "Command (from the repository root, on one MI355X), then the summary:", "", "```",
+           _TRACE_CMD.format(d=trace_dir), f"python bench.py --summarize-trace {trace_dir}", "```", "",
+           "Launches are grouped by kernel and launch shape (grid in work-items), so the full-size launches of a "
+           "pass are separate from the same kernel's smaller launches (plain per-coil path, frames, checks). "
+           "VGPR is the trace's VGPR_Count field.", "",
+           "| kernel | grid | block | VGPR | LDS B | calls | avg ms | min ms | max ms | % of GPU time |",
+           "|---|---|---:|---:|---:|---:|---:|---:|---:|---:|"]
+    for (name, grid, wg, vgpr, lds), ds in sorted(groups.items(), key=lambda kv: -sum(kv[1]))[:top]:
+        out.append(f"| `{name}` | {grid} | {wg} | {vgpr} | {lds} | {len(ds)} | {sum(ds) / len(ds):.4f} | "
+                   f"{min(ds):.4f} | {max(ds):.4f} | {100 * sum(ds) / total:.2f} |")
+    order = sorted(rows, key=lambda r: int(r["Start_Timestamp"]))
+    mark = max(i for i, r in enumerate(order) if CALIB_SUBSTR in r["Kernel_Name"])
+    timed = collections.OrderedDict()
+    for (tag, pat), r in zip(_TIMED_SEQ, order[mark + 1:mark + 1 + len(_TIMED_SEQ)]):
+        name = _short_kernel(r["Kernel_Name"])
+        if pat not in name:
+            raise SystemExit(f"unexpected launch {name} where {tag} was expected")
+        timed.setdefault((tag, name), []).append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6)
+    out += ["", "## Launches timed by bench.py's roofline", "",
+            "The launches that follow the calibration kernel (`torch.neg` over 1 GiB): an A^H A apply (P1-P5 and "
+            "the coil-sum partials), a type-2 and a type-1 transform at full size, three times over, which "
+            "bench.py times with HIP events on the library stream. Several passes share a kernel instance, so "
+            "each launch is labelled by its place in that sequence.", "",
+            "| pass | kernel | launches | avg ms |", "|---|---|---:|---:|"]
+    for (tag, name), ds in timed.items():
+        out.append(f"| {tag} | `{name}` | {len(ds)} | {sum(ds) / len(ds):.4f} |")
+    path = os.path.join(ROOT, "profiles", f"{workload}_kernel_stats.md")
+    with open(path, "w") as fh:
+        fh.write("\n".join(out) + "\n")
+    print(path)
+
+
+_SQ_PASSES = ("SQ_INSTS_VALU SQ_INSTS_SALU SQ_INSTS_SMEM SQ_INSTS_VMEM_RD SQ_WAIT_INST_ANY SQ_WAVE_CYCLES "
+              "SQ_BUSY_CYCLES SQ_ACTIVE_INST_VALU",
+              "SQ_WAVES SQ_INSTS_VALU_FMA_F32 SQ_INSTS_LDS SQ_LDS_BANK_CONFLICT SQ_ACTIVE_INST_LDS GRBM_GUI_ACTIVE")
+_SQ_KERNELS = (("spread_kernel_w6<c64, 32>", "spread_kernel_w6<HIP_vector_type<float, 2u>, 32>", 20_000_000),
+               ("interp_pair_kernel<6, 16, 16, 16>", "interp_pair_kernel<6, 16, 16, 16>", 40_000_000))
+
+
+def summarize_sq(dir_a, dir_b, workload):
+    """Markdown summary of the two SQ-counter passes over the NUFFT spread and
+    interpolation kernels (full-size launches only)."""
+    import collections
+    import csv
+    import glob
+    res = collections.defaultdict(dict)
+    for d in (dir_a, dir_b):
+        agg = collections.defaultdict(lambda: collections.defaultdict(list))
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection*.csv"), recursive=True):
+            for r in csv.DictReader(open(f)):
+                for label, sub, min_grid in _SQ_KERNELS:
+                    if sub in r["Kernel_Name"] and int(r["Grid_Size"]) >= min_grid:
+                        agg[label][r["Counter_Name"]].append(float(r["Counter_Value"]))
+        for label, counters in agg.items():
+            for c, vals in counters.items():
+                res[label][c] = sum(vals) / len(vals)
+    cmd = ("cd /tmp && export TMPDIR=/tmp && cd $REPO && rocprofv3 --pmc {c} --output-format csv -d {d} -o run "
+           "-- python -u bench.py --steps 1 --warmup 0 --no-cpu-baseline --phases type2,type1")
+    labels = [k for k, _, _ in _SQ_KERNELS if k in res]
+    out = [f"# SQ counters: NUFFT spread and interpolation, {workload}", "",
+           "Two counter passes (the hardware takes at most 8 SQ counters per pass), from the repository root on one "
+           "MI355X, then the summary:", "", "```", cmd.format(c=_SQ_PASSES[0], d=dir_a), cmd.format(c=_SQ_PASSES[1], d=dir_b),
+           f"python bench.py --summarize-sq {dir_a} {dir_b}", "```", "",
+           "Values are per launch (mean over the full-size launches), in units of 10^9. Instruction counts are "
+           "wave-instructions summed over the GPU; `GRBM_GUI_ACTIVE` is summed over the 8 XCDs.", "",
+           "| counter | " + " | ".join(f"`{k}`" for k in labels) + " |", "|---|" + "---:|" * len(labels)]
+    for c in sorted({c for k in labels for c in res[k]}):
+        out.append(f"| {c} | " + " | ".join(f"{res[k][c] / 1e9:.3f}" if c in res[k] else "" for k in labels) + " |")
+    out.append("")
+    for k in labels:
+        v, f = res[k].get("SQ_INSTS_VALU"), res[k].get("SQ_INSTS_VALU_FMA_F32")
+        if v and f:
+            out.append(f"- `{k}`: VALU-issue floor {v / 256 / 2.4e9 * 1e3:.1f} ms (one wave-instruction per CU per "
+                       f"cycle, 256 CUs, 2.4 GHz); FP32 FMAs are {100 * f / v:.0f}% of its VALU instructions.")
+    path = os.path.join(ROOT, "profiles", f"nufft_spread_interp_sq_counters_{workload}.md")
+    with open(path, "w") as fh:
+        fh.write("\n".join(out) + "\n")
+    print(path)
 
 
 def measured_traffic(workload, tag):
@@ -257,9 +380,21 @@ def main():
     ap.add_argument("--reduce-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="reduce two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of this script "
                          "into the traffic summary profiles/pmc_traffic_<workload>.json and exit")
+    ap.add_argument("--summarize-trace", metavar="TRACE_DIR",
+                    help="summarise a rocprofv3 --kernel-trace run of this script into "
+                         "profiles/<workload>_kernel_stats.md and exit")
+    ap.add_argument("--summarize-sq", nargs=2, metavar=("PASS1_DIR", "PASS2_DIR"),
+                    help="summarise the two SQ-counter passes over the NUFFT kernels into "
+                         "profiles/nufft_spread_interp_sq_counters_<workload>.md and exit")
     args = ap.parse_args()
     if args.reduce_pmc:
         reduce_pmc(args.reduce_pmc[0], args.reduce_pmc[1], args.workload)
+        return
+    if args.summarize_trace:
+        summarize_trace(args.summarize_trace, args.workload)
+        return
+    if args.summarize_sq:
+        summarize_sq(args.summarize_sq[0], args.summarize_sq[1], args.workload)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "

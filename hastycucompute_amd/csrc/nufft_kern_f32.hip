@@ -25,7 +25,7 @@ template <> void* nufft_spread_ptr<float2>(int cg) {
         case 4: return reinterpret_cast<void*>(&spread_kernel<float2, 4>);
         case 8: return reinterpret_cast<void*>(&spread_kernel<float2, 8>);
         case 16: return reinterpret_cast<void*>(&spread_kernel<float2, 16>);
-        case 32: return reinterpret_cast<void*>(&spread_kernel<float2, 32>);
+        case 32: return reinterpret_cast<void*>(&spread_kernel_w6<float2, 32>);
     }
     throw Error(HASTY_ERR_INTERNAL, "no spreading kernel for this transform group");
 }

@@ -322,9 +322,9 @@ __device__ inline bool arcs_meet(int a, int la, int b, int lb, int n) {
 // scalar loads.  csort holds CG - 1 padding elements past the
 // last point, so the CG values of a point are read without bounds tests.
 template <class V, int CG>
-__global__ void __launch_bounds__(kNufftThreads)
-    spread_kernel(GridArgs<V> a, const int4* __restrict__ visits, const V* __restrict__ csort,
-                  const decltype(V::x)* __restrict__ sw) {
+__device__ __forceinline__ void spread_body(const GridArgs<V>& a, const int4* __restrict__ visits,
+                                            const V* __restrict__ csort,
+                                            const decltype(V::x)* __restrict__ sw) {
     using R = decltype(V::x);
     int isub, igroup;
     if (!xcd_block(a.nsub, a.ngroups, a.xchunk, isub, igroup)) return;
@@ -424,6 +424,25 @@ __global__ void __launch_bounds__(kNufftThreads)
                 if (k < ng) dst[int64_t(k) * G] = cmake<V>(acc[2 * k], acc[2 * k + 1]);
         }
     }
+}
+
+template <class V, int CG>
+__global__ void __launch_bounds__(kNufftThreads)
+    spread_kernel(GridArgs<V> a, const int4* __restrict__ visits, const V* __restrict__ csort,
+                  const decltype(V::x)* __restrict__ sw) {
+    spread_body<V, CG>(a, visits, csort, sw);
+}
+
+// 32 complex64 transforms per block: the 64 accumulator VGPRs leave room
+// for 6 waves per SIMD only under an 80-VGPR budget (the compiler spills a
+// little outside the hit loop).  Measured at 128^3 x 32 coils: 24.4 ms at
+// 88 VGPRs / 5 waves, 23.4 ms at 80 / 6; at 7 waves the accumulators spill
+// inside the hit loop and it takes 46 ms.
+template <class V, int CG>
+__global__ void __launch_bounds__(kNufftThreads) __attribute__((amdgpu_waves_per_eu(6, 8)))
+    spread_kernel_w6(GridArgs<V> a, const int4* __restrict__ visits, const V* __restrict__ csort,
+                     const decltype(V::x)* __restrict__ sw) {
+    spread_body<V, CG>(a, visits, csort, sw);
 }
 
 // Sum the partial tiles of every split tile into the fine grid.
