@@ -460,8 +460,21 @@ __global__ void reduce_split_tiles_kernel(V* fine, const V* partial, const int4*
          idx += gridDim.x * blockDim.x) {
         const int b = idx / cells;
         const int c = idx - b * cells;
+        // slots in order; 8 loads in flight at a time (the kooshball centre
+        // splits into ~200 slots, which one dependent load at a time made
+        // latency-bound)
+        const V* src = partial + (int64_t(t.y) * nbatch + b) * cells + c;
+        const int64_t slot = int64_t(nbatch) * cells;
         V acc = cmake<V>(R(0), R(0));
-        for (int k = 0; k < t.z; ++k) acc = cadd(acc, partial[(int64_t(t.y + k) * nbatch + b) * cells + c]);
+        int k = 0;
+        for (; k + 8 <= t.z; k += 8) {
+            V v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = src[(k + u) * slot];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc = cadd(acc, v[u]);
+        }
+        for (; k < t.z; ++k) acc = cadd(acc, src[k * slot]);
         const int lx = c % g.T[2];
         const int ly = (c / g.T[2]) % g.T[1];
         const int lz = c / (g.T[2] * g.T[1]);
