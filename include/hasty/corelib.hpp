@@ -34,9 +34,11 @@
 #include <cmath>
 #include <complex>
 #include <cstdint>
+#include <filesystem>
 #include <fstream>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <optional>
 #include <stdexcept>
@@ -185,6 +187,111 @@ tensor<cpu_t, TT, RANK> import_binary_tensor(const std::string& path, std::array
     if (file.gcount() != want) throw std::runtime_error("Exception opening/reading file");
     return t;
 }
+
+// ---- cache_tensor (mod_tensor_caching.cppm:14-335): one host tensor under a
+// hash key, device copies made on first use and kept per device, and a
+// spill to / reload from `cache_dir / "<hash>.htc"` (the raw tensor file
+// above).  Copies of a cache_tensor share the same block.  get<cpu_t>()
+// returns a copy of the host data (the reference returns a torch view);
+// get_ptr<cpu_t>() shares it.
+enum struct device_idx {
+    CPU = -1, CUDA0 = 0, CUDA1, CUDA2, CUDA3, CUDA4, CUDA5, CUDA6, CUDA7, CUDA8, CUDA9, CUDA10, CUDA11,
+    CUDA12, CUDA13, CUDA14, CUDA15, MAX_CUDA_DEVICES = 16
+};
+
+inline std::filesystem::path cache_dir = std::filesystem::temp_directory_path();
+
+template <class TT, size_t RANK> class cache_tensor {
+    struct block {
+        std::mutex mutex;
+        std::shared_ptr<tensor<cpu_t, TT, RANK>> cpu;
+        std::array<int64_t, RANK> shape{};
+        size_t hashidx = 0;
+        std::array<std::shared_ptr<tensor<cuda_t, TT, RANK>>, size_t(device_idx::MAX_CUDA_DEVICES)> cuda;
+    };
+    std::shared_ptr<block> _block;
+
+    void require(const char* what) const {
+        if (!_block) throw std::runtime_error(std::string(what) + ": cache_tensor is not initialized");
+    }
+    std::filesystem::path file() const { return cache_dir / (std::to_string(_block->hashidx) + ".htc"); }
+    std::shared_ptr<tensor<cpu_t, TT, RANK>>& cpu_ptr() {
+        if (!_block->cpu)
+            _block->cpu = std::make_shared<tensor<cpu_t, TT, RANK>>(
+                import_binary_tensor<TT, RANK>(file().string(), _block->shape));
+        return _block->cpu;
+    }
+    std::shared_ptr<tensor<cuda_t, TT, RANK>>& cuda_ptr(device_idx idx) {
+        const int d = int(idx);
+        if (d < 0 || d >= int(device_idx::MAX_CUDA_DEVICES)) throw std::runtime_error("invalid device index");
+        auto& slot = _block->cuda[size_t(d)];
+        if (!slot) {
+            int prev = 0;
+            detail::check_hip(hipGetDevice(&prev), "hipGetDevice");
+            detail::check_hip(hipSetDevice(d), "hipSetDevice");
+            try {
+                slot = std::make_shared<tensor<cuda_t, TT, RANK>>(
+                    tensor<cuda_t, TT, RANK>::from_host(cpu_ptr()->vec(), _block->shape));
+            } catch (...) {
+                (void)hipSetDevice(prev);
+                throw;
+            }
+            detail::check_hip(hipSetDevice(prev), "hipSetDevice");
+        }
+        return slot;
+    }
+
+public:
+    cache_tensor() = default;
+    cache_tensor(tensor<cpu_t, TT, RANK>&& cputen, size_t hashidx) : _block(std::make_shared<block>()) {
+        _block->shape = cputen.shape_arr();
+        _block->cpu = std::make_shared<tensor<cpu_t, TT, RANK>>(std::move(cputen));
+        _block->hashidx = hashidx;
+    }
+    template <size_t R> int64_t shape() const {
+        static_assert(R < RANK, "shape index out of range");
+        require("shape");
+        return _block->shape[R];
+    }
+    std::array<int64_t, RANK> shape() const {
+        require("shape");
+        return _block->shape;
+    }
+    cache_tensor copy() const { return *this; }
+
+    template <class D> std::shared_ptr<tensor<D, TT, RANK>> get_ptr(device_idx idx = device_idx::CPU) {
+        require("get_ptr");
+        std::unique_lock<std::mutex> lock(_block->mutex);
+        if constexpr (std::is_same_v<D, cpu_t>) return cpu_ptr();
+        else return cuda_ptr(idx);
+    }
+    template <class D> tensor<D, TT, RANK> get(device_idx idx = device_idx::CPU) { return *get_ptr<D>(idx); }
+
+    void cache() {   // write the host data to cache_dir / "<hash>.htc"
+        require("cache");
+        std::unique_lock<std::mutex> lock(_block->mutex);
+        if (!_block->cpu) throw std::runtime_error("cache_disk: no tensor to cache");
+        export_binary_tensor(*_block->cpu, file().string());
+    }
+    void uncache() {  // reload the host data from the file
+        require("uncache");
+        std::unique_lock<std::mutex> lock(_block->mutex);
+        _block->cpu = std::make_shared<tensor<cpu_t, TT, RANK>>(
+            import_binary_tensor<TT, RANK>(file().string(), _block->shape));
+    }
+    void free_cpu() {
+        require("free_cpu");
+        std::unique_lock<std::mutex> lock(_block->mutex);
+        _block->cpu = nullptr;
+    }
+    void free(device_idx idx) {
+        require("free");
+        std::unique_lock<std::mutex> lock(_block->mutex);
+        const int d = int(idx);
+        if (d < 0 || d >= int(device_idx::MAX_CUDA_DEVICES)) throw std::runtime_error("invalid device index");
+        _block->cuda[size_t(d)] = nullptr;
+    }
+};
 
 template <class TT, size_t RANK>
 tensor<cuda_t, TT, RANK> make_empty_tensor(std::array<int64_t, RANK> shape) {
@@ -472,6 +579,13 @@ public:
     static constexpr size_t input_rank_t() { return DIM; }
     static constexpr size_t output_rank_t() { return DIM; }
 
+    // the reference's constructor (normal.cppm:55-60): the kernel and maps as
+    // cache_tensors, taken to the current device once
+    NORMAL_T_T1_OP(cache_tensor<c64_t, DIM>&& kernel, cache_tensor<c64_t, DIM + 1>&& stacked_diags,
+                   int coil_batch = 0)
+        : NORMAL_T_T1_OP(kernel.template get<cuda_t>(current_device()),
+                         stacked_diags.template get<cuda_t>(current_device()), coil_batch) {}
+
     NORMAL_T_T1_OP(tensor<cuda_t, c64_t, DIM>&& kernel, tensor<cuda_t, c64_t, DIM + 1>&& stacked_diags,
                    int coil_batch = 0)
         : _kernel(std::move(kernel)), _smaps(std::move(stacked_diags)) {
@@ -496,6 +610,11 @@ public:
     }
 
 private:
+    static device_idx current_device() {
+        int d = 0;
+        detail::check_hip(hipGetDevice(&d), "hipGetDevice");
+        return device_idx(d);
+    }
     tensor<cuda_t, c64_t, DIM> _kernel;
     tensor<cuda_t, c64_t, DIM + 1> _smaps;
     std::array<int64_t, DIM> _n{};

@@ -186,6 +186,25 @@ int main() {
     const auto hn = (*op)(dx).to_host();
     report("NORMAL_T_T1_OP vs NDFT normal operator", rel_l2(hn, ref_n), 2e-5);
 
+    // the reference's constructor: kernel and maps as cache_tensors (host data
+    // taken to the device once); the same operator, bit for bit
+    {
+        tensor<cpu_t, c64_t, 3> hk(kernel.shape_arr());
+        hk.vec() = kernel.to_host();
+        tensor<cpu_t, c64_t, 4> hs({p.C, p.n[0], p.n[1], p.n[2]});
+        hs.vec() = p.S;
+        cache_tensor<c64_t, 3> ck(std::move(hk), 101);
+        cache_tensor<c64_t, 4> cs(std::move(hs), 102);
+        NORMAL_T_T1_OP<cuda_t, c64_t, 3> cop(ck.copy(), cs.copy());
+        const auto hc = cop(dx).to_host();
+        bool same = hc.size() == hn.size();
+        for (size_t i = 0; same && i < hc.size(); ++i) same = hc[i] == hn[i];
+        report("NORMAL_T_T1_OP from cache_tensors vs from device tensors (0 = bitwise)", same ? 0.0 : 1.0, 0.5);
+        // the cached device copy is shared, not made again
+        const bool shared = ck.get_ptr<cuda_t>(device_idx::CUDA0).get() == ck.get_ptr<cuda_t>(device_idx::CUDA0).get();
+        report("cache_tensor keeps one device copy (0 = yes)", shared ? 0.0 : 1.0, 0.5);
+    }
+
     // toeplitz_multiplication with the reference's NORMAL_T_T1_OP modes, then sum
     {
         auto in = tensor<cuda_t, c64_t, 4>::from_host(p.S, {p.C, p.n[0], p.n[1], p.n[2]});

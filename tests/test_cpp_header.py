@@ -57,3 +57,47 @@ int main(int argc, char** argv) {
     t = import_binary_tensor(out, (3, 4), torch.complex64)
     ref = torch.complex(torch.arange(12.0), -0.5 * torch.arange(12.0)).reshape(3, 4)
     assert torch.equal(t, ref)
+
+
+def test_cache_tensor_host_only(tmp_path):
+    """cache_tensor of the C++ mirror (mod_tensor_caching.cppm): shape, shared
+    copies, spill to cache_dir / "<hash>.htc" (the raw file the Python mirror
+    reads), reload after free_cpu, and the reference's error messages."""
+    src = tmp_path / "ct.cpp"
+    src.write_text(r'''
+#include <hasty/corelib.hpp>
+using namespace hasty;
+int main(int argc, char** argv) {
+    cache_dir = argv[1];
+    tensor<cpu_t, c64_t, 2> t({2, 3});
+    for (int i = 0; i < 6; ++i) t.vec()[size_t(i)] = c64_t(float(i), 1.5f * float(i));
+    cache_tensor<c64_t, 2> c(std::move(t), 4242);
+    if (c.shape<0>() != 2 || c.shape()[1] != 3) return 2;
+    auto d = c.copy();
+    c.cache();
+    c.free_cpu();
+    auto h = d.get<cpu_t>();                       // shared block: reloaded from the file
+    for (int i = 0; i < 6; ++i) if (h.vec()[size_t(i)] != c64_t(float(i), 1.5f * float(i))) return 3;
+    if (c.get_ptr<cpu_t>().get() != d.get_ptr<cpu_t>().get()) return 4;
+    c.uncache();
+    if (c.get<cpu_t>().vec()[5] != c64_t(5.f, 7.5f)) return 5;
+    cache_tensor<c64_t, 2> empty;
+    try { (void)empty.shape(); return 6; }
+    catch (const std::runtime_error& e) { if (std::string(e.what()) != "shape: cache_tensor is not initialized") return 7; }
+    cache_tensor<c64_t, 1> none(tensor<cpu_t, c64_t, 1>({4}), 77);
+    try { none.free_cpu(); (void)none.get<cpu_t>(); return 8; }
+    catch (const std::runtime_error& e) { if (std::string(e.what()) != "Could not open file for reading") return 9; }
+    return 0;
+}
+''')
+    exe = tmp_path / "ct"
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-D__HIP_PLATFORM_AMD__",
+                        "-I", os.path.join(ROOT, "include"), "-I", "/opt/rocm/include", str(src), "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe), str(tmp_path)], capture_output=True, text=True)
+    assert r.returncode == 0, (r.returncode, r.stderr)
+    import torch
+    from hastycucompute_amd.io import import_binary_tensor
+    back = import_binary_tensor(tmp_path / "4242.htc", (2, 3), torch.complex64)
+    assert torch.equal(back, torch.complex(torch.arange(6.0), 1.5 * torch.arange(6.0)).reshape(2, 3))
