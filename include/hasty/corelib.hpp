@@ -293,6 +293,14 @@ public:
     }
 };
 
+namespace detail {
+inline device_idx current_device() {
+    int d = 0;
+    check_hip(hipGetDevice(&d), "hipGetDevice");
+    return device_idx(d);
+}
+}  // namespace detail
+
 template <class TT, size_t RANK>
 tensor<cuda_t, TT, RANK> make_empty_tensor(std::array<int64_t, RANK> shape) {
     return tensor<cuda_t, TT, RANK>::empty(shape);
@@ -583,8 +591,8 @@ public:
     // cache_tensors, taken to the current device once
     NORMAL_T_T1_OP(cache_tensor<c64_t, DIM>&& kernel, cache_tensor<c64_t, DIM + 1>&& stacked_diags,
                    int coil_batch = 0)
-        : NORMAL_T_T1_OP(kernel.template get<cuda_t>(current_device()),
-                         stacked_diags.template get<cuda_t>(current_device()), coil_batch) {}
+        : NORMAL_T_T1_OP(kernel.template get<cuda_t>(detail::current_device()),
+                         stacked_diags.template get<cuda_t>(detail::current_device()), coil_batch) {}
 
     NORMAL_T_T1_OP(tensor<cuda_t, c64_t, DIM>&& kernel, tensor<cuda_t, c64_t, DIM + 1>&& stacked_diags,
                    int coil_batch = 0)
@@ -610,11 +618,6 @@ public:
     }
 
 private:
-    static device_idx current_device() {
-        int d = 0;
-        detail::check_hip(hipGetDevice(&d), "hipGetDevice");
-        return device_idx(d);
-    }
     tensor<cuda_t, c64_t, DIM> _kernel;
     tensor<cuda_t, c64_t, DIM + 1> _smaps;
     std::array<int64_t, DIM> _n{};
@@ -886,6 +889,52 @@ tensor<cuda_t, TT, DIM + 1> sense_adjoint_offresonance(const std::array<tensor<c
     detail::check(hasty_sense_adjoint(plan->handle(), int(C), int(bil.template shape<0>()), kdata.data_ptr(),
                                       smaps.data_ptr(), bil.data_ptr(), cjl.data_ptr(), out.mutable_data_ptr(), 0));
     return out;
+}
+
+// The reference's form (forward.cppm:35-46, adjoint.cppm:33-43): cache_tensor
+// arguments and a host result.  The coils run batched on the current device
+// (there is no thread pool); free_bil / free_cjl drop the device copies of
+// bil / cjl afterwards, and the plan lives for the call either way.
+template <class D, class TT, class TTC, size_t DIM>
+tensor<cpu_t, TT, 2> sense_forward_offresonance(std::array<cache_tensor<TTC, 1>, DIM>& coords,
+                                                cache_tensor<TT, DIM>& image, cache_tensor<TT, DIM + 1>& smaps,
+                                                cache_tensor<TT, 2>& bil, cache_tensor<TT, DIM + 1>& cjl,
+                                                bool free_bil = true, bool free_cjl = true,
+                                                bool free_nufft_plan = true) {
+    static_assert(std::is_same_v<D, cuda_t>, "the time-segmented operators run on the GPU");
+    (void)free_nufft_plan;
+    const device_idx d = detail::current_device();
+    std::array<tensor<cuda_t, TTC, 1>, DIM> dc;
+    for (size_t a = 0; a < DIM; ++a) dc[a] = coords[a].template get<cuda_t>(d);
+    auto out = sense_forward_offresonance<TT, TTC, DIM>(dc, image.template get<cuda_t>(d),
+                                                        smaps.template get<cuda_t>(d), bil.template get<cuda_t>(d),
+                                                        cjl.template get<cuda_t>(d));
+    if (free_bil) bil.free(d);
+    if (free_cjl) cjl.free(d);
+    tensor<cpu_t, TT, 2> h(out.shape_arr());
+    h.vec() = out.to_host();
+    return h;
+}
+
+template <class D, class TT, class TTC, size_t DIM>
+tensor<cpu_t, TT, DIM + 1> sense_adjoint_offresonance(std::array<cache_tensor<TTC, 1>, DIM>& coords,
+                                                      cache_tensor<TT, 2>& kdata, cache_tensor<TT, DIM + 1>& smaps,
+                                                      cache_tensor<TT, 2>& bil, cache_tensor<TT, DIM + 1>& cjl,
+                                                      bool free_bil = true, bool free_cjl = true,
+                                                      bool free_nufft_plan = true) {
+    static_assert(std::is_same_v<D, cuda_t>, "the time-segmented operators run on the GPU");
+    (void)free_nufft_plan;
+    const device_idx d = detail::current_device();
+    std::array<tensor<cuda_t, TTC, 1>, DIM> dc;
+    for (size_t a = 0; a < DIM; ++a) dc[a] = coords[a].template get<cuda_t>(d);
+    auto out = sense_adjoint_offresonance<TT, TTC, DIM>(dc, kdata.template get<cuda_t>(d),
+                                                        smaps.template get<cuda_t>(d), bil.template get<cuda_t>(d),
+                                                        cjl.template get<cuda_t>(d));
+    if (free_bil) bil.free(d);
+    if (free_cjl) cjl.free(d);
+    tensor<cpu_t, TT, DIM + 1> h(out.shape_arr());
+    h.vec() = out.to_host();
+    return h;
 }
 
 // ---- CG (cg.cppm:122-184) and power iteration (alg.cppm:13-30) -----------------

@@ -300,6 +300,25 @@ int main() {
             back[size_t(i)] *= std::conj(std::complex<double>(p.S[size_t(i)]));
         std::vector<c64_t> x0(xa.begin(), xa.begin() + I);
         report("sense_adjoint_offresonance coil 0 vs NDFT", rel_l2(x0, back), 2e-5);
+
+        // the reference's form: cache_tensor arguments, host results, the same numbers
+        auto host = [](const auto& dt, size_t h) {
+            constexpr size_t R = std::tuple_size_v<std::decay_t<decltype(dt.shape_arr())>>;
+            using E = std::decay_t<decltype(*dt.data_ptr())>;
+            tensor<cpu_t, E, R> t(dt.shape_arr());
+            t.vec() = dt.to_host();
+            return cache_tensor<E, R>(std::move(t), h);
+        };
+        std::array<cache_tensor<f32_t, 1>, 3> ccoords{host(coords[0], 201), host(coords[1], 202), host(coords[2], 203)};
+        auto cimg = host(dx, 204);
+        auto cmaps = host(maps, 205);
+        auto cbil = host(bil, 206);
+        auto ccjl = host(cjl, 207);
+        auto ckd = host(dk, 208);
+        const auto yc = sense_forward_offresonance<cuda_t, c64_t, f32_t, 3>(ccoords, cimg, cmaps, cbil, ccjl);
+        const auto xc = sense_adjoint_offresonance<cuda_t, c64_t, f32_t, 3>(ccoords, ckd, cmaps, cbil, ccjl);
+        report("sense_*_offresonance from cache_tensors vs device tensors (0 = bitwise)",
+               (yc.vec() == y && xc.vec() == xa) ? 0.0 : 1.0, 0.5);
     }
 
     // CG on the normal equations: 30 iterations reduce the residual by 1e3
