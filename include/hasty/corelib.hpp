@@ -299,6 +299,10 @@ inline device_idx current_device() {
     check_hip(hipGetDevice(&d), "hipGetDevice");
     return device_idx(d);
 }
+// a cache_tensor's copy on the current device (made once, then shared)
+template <class TT, size_t RANK> tensor<cuda_t, TT, RANK> on_device(cache_tensor<TT, RANK>& c) {
+    return c.template get<cuda_t>(current_device());
+}
 }  // namespace detail
 
 template <class TT, size_t RANK>
@@ -765,6 +769,11 @@ public:
                                  tensor<cuda_t, c64_t, 2>(), coil_batch) {
         (void)store_module;
     }
+    // the reference's constructor (normal.cppm:206-210)
+    NORMAL_IDT_T1_OP(cache_tensor<c64_t, DIM + 1>&& kernels, cache_tensor<c64_t, DIM + 1>&& kerneldiags,
+                     cache_tensor<c64_t, DIM + 1>&& stacked_diags, bool store_module = false, int coil_batch = 0)
+        : NORMAL_IDT_T1_OP(detail::on_device(kernels), detail::on_device(kerneldiags), detail::on_device(stacked_diags),
+                           store_module, coil_batch) {}
 };
 
 // normal.cppm:405-564; the output-side maps S'_c = sum_i conj(W_ic) S_i are formed once
@@ -778,6 +787,12 @@ public:
                                  std::move(weights), coil_batch) {
         (void)store_module;
     }
+    // the reference's constructor (normal.cppm:424-430)
+    NORMAL_IDTW_T1_OP(cache_tensor<c64_t, DIM + 1>&& kernels, cache_tensor<c64_t, DIM + 1>&& kerneldiags,
+                      cache_tensor<c64_t, DIM + 1>&& stacked_diags, cache_tensor<c64_t, 2>&& weights,
+                      bool store_module = false, int coil_batch = 0)
+        : NORMAL_IDTW_T1_OP(detail::on_device(kernels), detail::on_device(kerneldiags),
+                            detail::on_device(stacked_diags), detail::on_device(weights), store_module, coil_batch) {}
 };
 
 // normal.cppm:113-160
@@ -787,6 +802,9 @@ public:
     NORMAL_T_T2_OP(tensor<cuda_t, c64_t, DIM>&& kernel, tensor<cuda_t, c64_t, DIM>&& x, int stack_batch = 0)
         : detail::normal_t2<DIM>(as_stack(std::move(kernel)), tensor<cuda_t, c64_t, DIM + 1>(), std::move(x),
                                  tensor<cuda_t, c64_t, 2>(), stack_batch) {}
+    // the reference's constructor (normal.cppm:124-126)
+    NORMAL_T_T2_OP(cache_tensor<c64_t, DIM>&& kernel, cache_tensor<c64_t, DIM>&& x, int stack_batch = 0)
+        : NORMAL_T_T2_OP(detail::on_device(kernel), detail::on_device(x), stack_batch) {}
 
 private:
     static tensor<cuda_t, c64_t, DIM + 1> as_stack(tensor<cuda_t, c64_t, DIM>&& k) {
@@ -808,6 +826,11 @@ public:
                      tensor<cuda_t, c64_t, DIM>&& x, int stack_batch = 0)
         : detail::normal_t2<DIM>(std::move(kernels), std::move(kerneldiags), std::move(x),
                                  tensor<cuda_t, c64_t, 2>(), stack_batch) {}
+    // the reference's constructor (normal.cppm:311-314)
+    NORMAL_IDT_T2_OP(cache_tensor<c64_t, DIM + 1>&& kernels, cache_tensor<c64_t, DIM + 1>&& kerneldiags,
+                     cache_tensor<c64_t, DIM>&& x, int stack_batch = 0)
+        : NORMAL_IDT_T2_OP(detail::on_device(kernels), detail::on_device(kerneldiags), detail::on_device(x),
+                           stack_batch) {}
 };
 
 // normal.cppm:586-735
@@ -821,6 +844,12 @@ public:
                                  stack_batch) {
         (void)store_module;
     }
+    // the reference's constructor (normal.cppm:602-607)
+    NORMAL_IDTW_T2_OP(cache_tensor<c64_t, DIM + 1>&& kernels, cache_tensor<c64_t, DIM + 1>&& kerneldiags,
+                      cache_tensor<c64_t, DIM>&& x, cache_tensor<c64_t, 2>&& weights, bool store_module = false,
+                      int stack_batch = 0)
+        : NORMAL_IDTW_T2_OP(detail::on_device(kernels), detail::on_device(kerneldiags), detail::on_device(x),
+                            detail::on_device(weights), store_module, stack_batch) {}
 };
 
 // ---- time-segmented SENSE (forward.cppm:36-165, adjoint.cppm:33-171) -----------

@@ -275,6 +275,33 @@ int main() {
             lhs += std::conj(std::complex<double>(p.S[size_t(i)])) * std::complex<double>(ht2[size_t(i)]);
         for (int64_t i = 0; i < I; ++i) rhs += std::conj(std::complex<double>(p.x[size_t(i)])) * ref_n[size_t(i)];
         report("NORMAL_T_T2_OP: <S, T2(S)> vs <x, N x>", std::abs(lhs - rhs) / std::abs(rhs), 2e-5);
+
+        // the reference's cache_tensor constructors build the same operators
+        auto cache = [](const std::vector<c64_t>& v, auto shape, size_t h) {
+            tensor<cpu_t, c64_t, std::tuple_size_v<decltype(shape)>> t(shape);
+            t.vec() = v;
+            return cache_tensor<c64_t, std::tuple_size_v<decltype(shape)>>(std::move(t), h);
+        };
+        const std::array<int64_t, 3> kshape3{2 * p.n[0], 2 * p.n[1], 2 * p.n[2]};
+        const std::array<int64_t, 2> wshape{p.C, p.C};
+        NORMAL_IDT_T1_OP<cuda_t, c64_t, 3> idt_c(cache(kk, kshape, 301), cache(dd, dshape, 302),
+                                                 cache(p.S, sshape, 303));
+        NORMAL_IDTW_T1_OP<cuda_t, c64_t, 3> idtw_c(cache(kk, kshape, 301), cache(dd, dshape, 302),
+                                                   cache(p.S, sshape, 303), cache(W, wshape, 304));
+        NORMAL_T_T2_OP<cuda_t, c64_t, 3> t2_c(cache(hkern, kshape3, 305), cache(p.x, p.n, 306));
+        NORMAL_IDT_T2_OP<cuda_t, c64_t, 3> idt2(tensor<cuda_t, c64_t, 4>::from_host(kk, kshape),
+                                                tensor<cuda_t, c64_t, 4>::from_host(dd, dshape), dx.clone());
+        NORMAL_IDT_T2_OP<cuda_t, c64_t, 3> idt2_c(cache(kk, kshape, 301), cache(dd, dshape, 302),
+                                                  cache(p.x, p.n, 306));
+        NORMAL_IDTW_T2_OP<cuda_t, c64_t, 3> idtw2(tensor<cuda_t, c64_t, 4>::from_host(kk, kshape),
+                                                  tensor<cuda_t, c64_t, 4>::from_host(dd, dshape), dx.clone(),
+                                                  tensor<cuda_t, c64_t, 2>::from_host(W, {p.C, p.C}));
+        NORMAL_IDTW_T2_OP<cuda_t, c64_t, 3> idtw2_c(cache(kk, kshape, 301), cache(dd, dshape, 302),
+                                                    cache(p.x, p.n, 306), cache(W, wshape, 304));
+        const bool same = idt_c(dx).to_host() == idt(dx).to_host() && idtw_c(dx).to_host() == idtw(dx).to_host() &&
+                          t2_c(psi).to_host() == ht2 && idt2_c(psi).to_host() == idt2(psi).to_host() &&
+                          idtw2_c(psi).to_host() == idtw2(psi).to_host();
+        report("normal operators from cache_tensors vs device tensors (0 = bitwise)", same ? 0.0 : 1.0, 0.5);
         expect_throw("kernel / diagonal count mismatch", "Number of kernels and kernel diagonals must match.", [&] {
             NORMAL_IDT_T1_OP<cuda_t, c64_t, 3> bad(tensor<cuda_t, c64_t, 4>::from_host(kk, kshape),
                                                    tensor<cuda_t, c64_t, 4>::from_host(R, {1, p.n[0], p.n[1], p.n[2]}),

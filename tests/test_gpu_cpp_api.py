@@ -1,7 +1,8 @@
 """GPU: the C++ corelib mirror (include/hasty/corelib.hpp) driven by the
 program tests/cpp/test_corelib.cpp, which checks nufft_plan type 1 / type 2,
 nufft_backward_cpu_over_cuda, toeplitz_kernel, toeplitz_multiply,
-fft::toeplitz_multiplication, NORMAL_T_T1_OP, conjugate_gradient and max_eig
+fft::toeplitz_multiplication, the normal-operator family (also built from
+cache_tensors), time-segmented SENSE, conjugate_gradient and max_eig
 against a direct double-precision NDFT, and the reference's error messages."""
 import os
 import subprocess
