@@ -48,6 +48,7 @@
 #include <vector>
 
 #include "../hasty_hip.h"
+#include "threading.hpp"
 
 namespace hasty {
 
@@ -867,6 +868,7 @@ uptr<nufft_plan<cuda_t, TTC, DIM, NT>> sense_plan(const std::array<tensor<cuda_t
     o.ntransf = int32_t(coil_batch > 0 ? std::min<int64_t>(coil_batch, C) : C);
     o.tol = std::is_same_v<TTC, f32_t> ? TTC(1e-6) : TTC(1e-13);
     o.sign = NT == nufft_type::TYPE_1 ? nufft_sign::DEFAULT_TYPE_1 : nufft_sign::DEFAULT_TYPE_2;
+    o.device_idx = int32_t(current_device());
     auto plan = nufft_plan<cuda_t, TTC, DIM, NT>::make(o);
     plan->setpts(coords);
     return plan;
@@ -964,6 +966,176 @@ tensor<cpu_t, TT, DIM + 1> sense_adjoint_offresonance(std::array<cache_tensor<TT
     tensor<cpu_t, TT, DIM + 1> h(out.shape_arr());
     h.vec() = out.to_host();
     return h;
+}
+
+// The reference's thread-pool form (forward.cppm:35-46, adjoint.cppm:33-43).
+// The coils are split into one contiguous chunk per pool worker.  A worker
+// runs on the device named by the device_idx entry "device_idx" of its
+// storage and pushes its chunk through one plan of ntransf = chunk size,
+// kept in its storage as "nufft_plan" until free_nufft_plan clears it (a kept
+// plan is reused while the chunk size matches, whatever the coordinates, as in
+// the reference).  The chunk's maps go to the device per call; the image, bil,
+// cjl and coordinates come from the cache_tensors' device copies, and
+// free_bil / free_cjl drop those of bil / cjl on every device used.  Chunks
+// run concurrently on distinct devices; workers that share a device share its
+// null stream.
+namespace detail {
+template <class NP> struct pooled_plan {
+    uptr<NP> plan;
+    int64_t ntransf = 0;
+};
+template <class TTC, size_t DIM, nufft_type NT>
+nufft_plan<cuda_t, TTC, DIM, NT>& pool_plan(threading::storage& store, std::array<cache_tensor<TTC, 1>, DIM>& coords,
+                                             const std::array<int64_t, DIM>& n, int64_t nt, device_idx d) {
+    using E = pooled_plan<nufft_plan<cuda_t, TTC, DIM, NT>>;
+    auto e = store.get_ptr<E>("nufft_plan");
+    if (!e || e->ntransf != nt) {
+        store.clear("nufft_plan");
+        std::array<tensor<cuda_t, TTC, 1>, DIM> dc;
+        for (size_t a = 0; a < DIM; ++a) dc[a] = coords[a].template get<cuda_t>(d);
+        e = std::make_shared<E>();
+        e->plan = sense_plan<TTC, DIM, NT>(dc, n, nt, 0);
+        e->ntransf = nt;
+        store.add<E>("nufft_plan", e);
+    }
+    return *e->plan;
+}
+// runs body(c0, c1, device) for contiguous coil chunks [c0, c1) on the pool;
+// returns the devices used
+template <class F>
+std::vector<device_idx> run_coil_chunks(threading::storage_thread_pool_interface& pool, int64_t C, F body) {
+    const int64_t K = std::max<int64_t>(1, std::min<int64_t>(C, int64_t(pool.nthreads())));
+    const int64_t per = (C + K - 1) / std::max<int64_t>(K, 1);
+    std::mutex m;
+    std::vector<device_idx> used;
+    std::vector<std::future<void>> futs;
+    try {
+        for (int64_t c0 = 0; c0 < C; c0 += per) {
+            const int64_t c1 = std::min(C, c0 + per);
+            futs.push_back(pool.enqueue(
+                [&, c0, c1](threading::storage& store) {
+                    const device_idx d = store.get_ref_throw<device_idx>("device_idx");
+                    check_hip(hipSetDevice(int(d)), "hipSetDevice");
+                    {
+                        std::lock_guard<std::mutex> g(m);
+                        if (std::find(used.begin(), used.end(), d) == used.end()) used.push_back(d);
+                    }
+                    body(c0, c1, d, store);
+                },
+                {}));
+        }
+    } catch (...) {  // the queued chunks use this frame: let them finish first
+        for (auto& f : futs) f.wait();
+        throw;
+    }
+    std::exception_ptr first;
+    for (auto& f : futs) {  // wait for every chunk before reporting the first failure
+        try {
+            f.get();
+        } catch (...) {
+            if (!first) first = std::current_exception();
+        }
+    }
+    if (first) std::rethrow_exception(first);
+    return used;
+}
+template <class TT, size_t R> void free_on(cache_tensor<TT, R>& t, const std::vector<device_idx>& used) {
+    for (auto d : used) t.free(d);
+}
+}  // namespace detail
+
+template <class D, class TT, class TTC, size_t DIM>
+tensor<cpu_t, TT, 2> sense_forward_offresonance(std::array<cache_tensor<TTC, 1>, DIM>& coords,
+                                                cache_tensor<TT, DIM>& image, cache_tensor<TT, DIM + 1>& smaps,
+                                                cache_tensor<TT, 2>& bil, cache_tensor<TT, DIM + 1>& cjl,
+                                                threading::storage_thread_pool_interface& thread_pool,
+                                                bool free_bil = true, bool free_cjl = true,
+                                                bool free_nufft_plan = true) {
+    static_assert(std::is_same_v<D, cuda_t>, "the time-segmented operators run on the GPU");
+    static_assert(std::is_same_v<TT, complex_t<TTC>>, "data must be the plan's complex type");
+    const auto sshape = smaps.shape();
+    const int64_t C = sshape[0], M = coords[0].template shape<0>();
+    std::array<int64_t, DIM> n{};
+    for (size_t a = 0; a < DIM; ++a) n[a] = sshape[a + 1];
+    if (image.shape() != n) throw std::runtime_error("image must have the maps' spatial shape");
+    const int64_t I = std::accumulate(n.begin(), n.end(), int64_t(1), std::multiplies<int64_t>());
+    const int64_t L = bil.template shape<0>();
+    if (bil.template shape<1>() != M || cjl.template shape<0>() != L)
+        throw std::runtime_error("bil must be [L, M] and cjl [L, N...]");
+    tensor<cpu_t, TT, 2> out({C, M});
+    auto hmaps = smaps.template get_ptr<cpu_t>();
+    auto used = detail::run_coil_chunks(thread_pool, C, [&](int64_t c0, int64_t c1, device_idx d,
+                                                            threading::storage& store) {
+        const int64_t nc = c1 - c0;
+        auto& plan = detail::pool_plan<TTC, DIM, nufft_type::TYPE_2>(store, coords, n, nc, d);
+        std::array<int64_t, DIM + 1> cs{};
+        cs[0] = nc;
+        for (size_t a = 0; a < DIM; ++a) cs[a + 1] = n[a];
+        auto maps = tensor<cuda_t, TT, DIM + 1>::empty(cs);
+        detail::check_hip(hipMemcpy(maps.mutable_data_ptr(), hmaps->data_ptr() + c0 * I, size_t(nc * I) * sizeof(TT),
+                                    hipMemcpyHostToDevice), "hipMemcpy H2D");
+        auto y = tensor<cuda_t, TT, 2>::empty({nc, M});
+        auto img = image.template get<cuda_t>(d);
+        auto b = bil.template get<cuda_t>(d);
+        auto cj = cjl.template get<cuda_t>(d);
+        detail::check(hasty_sense_forward(plan.handle(), int(nc), int(L), img.data_ptr(), maps.data_ptr(),
+                                          b.data_ptr(), cj.data_ptr(), y.mutable_data_ptr()));
+        detail::check_hip(hipMemcpy(out.mutable_data_ptr() + c0 * M, y.data_ptr(), size_t(nc * M) * sizeof(TT),
+                                    hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    });
+    if (free_bil) detail::free_on(bil, used);
+    if (free_cjl) detail::free_on(cjl, used);
+    if (free_nufft_plan) thread_pool.clear_from_storages({"nufft_plan"});
+    return out;
+}
+
+template <class D, class TT, class TTC, size_t DIM>
+tensor<cpu_t, TT, DIM + 1> sense_adjoint_offresonance(std::array<cache_tensor<TTC, 1>, DIM>& coords,
+                                                      cache_tensor<TT, 2>& kdata, cache_tensor<TT, DIM + 1>& smaps,
+                                                      cache_tensor<TT, 2>& bil, cache_tensor<TT, DIM + 1>& cjl,
+                                                      threading::storage_thread_pool_interface& thread_pool,
+                                                      bool free_bil = true, bool free_cjl = true,
+                                                      bool free_nufft_plan = true) {
+    static_assert(std::is_same_v<D, cuda_t>, "the time-segmented operators run on the GPU");
+    static_assert(std::is_same_v<TT, complex_t<TTC>>, "data must be the plan's complex type");
+    const auto sshape = smaps.shape();
+    const int64_t C = sshape[0], M = coords[0].template shape<0>();
+    std::array<int64_t, DIM> n{};
+    for (size_t a = 0; a < DIM; ++a) n[a] = sshape[a + 1];
+    if (kdata.template shape<0>() != C || kdata.template shape<1>() != M)
+        throw std::runtime_error("kdata must be [C, M]");
+    const int64_t I = std::accumulate(n.begin(), n.end(), int64_t(1), std::multiplies<int64_t>());
+    const int64_t L = bil.template shape<0>();
+    if (bil.template shape<1>() != M || cjl.template shape<0>() != L)
+        throw std::runtime_error("bil must be [L, M] and cjl [L, N...]");
+    tensor<cpu_t, TT, DIM + 1> out(sshape);
+    auto hmaps = smaps.template get_ptr<cpu_t>();
+    auto hk = kdata.template get_ptr<cpu_t>();
+    auto used = detail::run_coil_chunks(thread_pool, C, [&](int64_t c0, int64_t c1, device_idx d,
+                                                            threading::storage& store) {
+        const int64_t nc = c1 - c0;
+        auto& plan = detail::pool_plan<TTC, DIM, nufft_type::TYPE_1>(store, coords, n, nc, d);
+        std::array<int64_t, DIM + 1> cs{};
+        cs[0] = nc;
+        for (size_t a = 0; a < DIM; ++a) cs[a + 1] = n[a];
+        auto maps = tensor<cuda_t, TT, DIM + 1>::empty(cs);
+        auto k = tensor<cuda_t, TT, 2>::empty({nc, M});
+        detail::check_hip(hipMemcpy(maps.mutable_data_ptr(), hmaps->data_ptr() + c0 * I, size_t(nc * I) * sizeof(TT),
+                                    hipMemcpyHostToDevice), "hipMemcpy H2D");
+        detail::check_hip(hipMemcpy(k.mutable_data_ptr(), hk->data_ptr() + c0 * M, size_t(nc * M) * sizeof(TT),
+                                    hipMemcpyHostToDevice), "hipMemcpy H2D");
+        auto x = tensor<cuda_t, TT, DIM + 1>::empty(cs);
+        auto b = bil.template get<cuda_t>(d);
+        auto cj = cjl.template get<cuda_t>(d);
+        detail::check(hasty_sense_adjoint(plan.handle(), int(nc), int(L), k.data_ptr(), maps.data_ptr(),
+                                          b.data_ptr(), cj.data_ptr(), x.mutable_data_ptr(), 0));
+        detail::check_hip(hipMemcpy(out.mutable_data_ptr() + c0 * I, x.data_ptr(), size_t(nc * I) * sizeof(TT),
+                                    hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    });
+    if (free_bil) detail::free_on(bil, used);
+    if (free_cjl) detail::free_on(cjl, used);
+    if (free_nufft_plan) thread_pool.clear_from_storages({"nufft_plan"});
+    return out;
 }
 
 // ---- CG (cg.cppm:122-184) and power iteration (alg.cppm:13-30) -----------------

@@ -346,6 +346,31 @@ int main() {
         const auto xc = sense_adjoint_offresonance<cuda_t, c64_t, f32_t, 3>(ccoords, ckd, cmaps, cbil, ccjl);
         report("sense_*_offresonance from cache_tensors vs device tensors (0 = bitwise)",
                (yc.vec() == y && xc.vec() == xa) ? 0.0 : 1.0, 0.5);
+
+        // the thread-pool form: two workers on this device split the 3 coils
+        // into chunks of 2 and 1; a kept plan stays in the storages until cleared
+        std::vector<threading::storage> st(2);
+        for (auto& s : st) s.add<device_idx>("device_idx", std::make_shared<device_idx>(device_idx::CUDA0));
+        auto pool = std::make_shared<threading::storage_thread_pool>(std::move(st));
+        threading::storage_thread_pool_interface iface(pool, {"device_idx"});
+        const auto yp = sense_forward_offresonance<cuda_t, c64_t, f32_t, 3>(ccoords, cimg, cmaps, cbil, ccjl, iface,
+                                                                            false, false, false);
+        bool kept = true;
+        for (const auto& s : pool->storages()) kept &= s.exist("nufft_plan");
+        const auto yp2 = sense_forward_offresonance<cuda_t, c64_t, f32_t, 3>(ccoords, cimg, cmaps, cbil, ccjl, iface);
+        bool cleared = true;
+        for (const auto& s : pool->storages()) cleared &= !s.exist("nufft_plan");
+        const auto xp = sense_adjoint_offresonance<cuda_t, c64_t, f32_t, 3>(ccoords, ckd, cmaps, cbil, ccjl, iface);
+        report("sense_forward_offresonance on a 2-worker pool vs batched", rel_l2(yp.vec(), yc.vec()), 1e-6);
+        report("sense_forward_offresonance with the kept plan (0 = bitwise)", yp2.vec() == yp.vec() ? 0.0 : 1.0, 0.5);
+        report("sense_adjoint_offresonance on a 2-worker pool vs batched", rel_l2(xp.vec(), xc.vec()), 1e-6);
+        report("pool keeps the plan, then free_nufft_plan clears it (0 = yes)", kept && cleared ? 0.0 : 1.0, 0.5);
+        expect_throw("pool worker without a device_idx", "Storage does not contain key: device_idx", [&] {
+            std::vector<threading::storage> bare(1);
+            threading::storage_thread_pool_interface none(
+                std::make_shared<threading::storage_thread_pool>(std::move(bare)));
+            (void)sense_forward_offresonance<cuda_t, c64_t, f32_t, 3>(ccoords, cimg, cmaps, cbil, ccjl, none);
+        });
     }
 
     // CG on the normal equations: 30 iterations reduce the residual by 1e3
