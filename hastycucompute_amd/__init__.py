@@ -13,6 +13,7 @@ The Python modules mirror the reference's corelib operator API:
 * ``hastycucompute_amd.offres`` -- time-segmentation interpolators
 * ``hastycucompute_amd.io`` -- raw tensor files (``.htc``)
 * ``hastycucompute_amd.hdf5`` -- HDF5 tensor files and the raw k-space loader
+* ``hastycucompute_amd.generic_value`` -- the generic_value wire format
 * ``hastycucompute_amd.ops`` -- ``torch.ops.hasty.*`` registrations
 * ``hastycucompute_amd.autograd``, ``frames``, ``autotune``, ``trajectory``
 """
