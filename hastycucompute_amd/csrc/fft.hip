@@ -98,6 +98,7 @@ static bool force_stockham() {
     }();
     return v;
 }
+bool fft_force_stockham() { return force_stockham(); }
 
 template <class V>
 static PassArgs<V> blank_pass() {
@@ -119,8 +120,10 @@ void launch_row_load(RowLoadArgs<V> a, hipStream_t s) {
         p.lm1 = a.m1; p.lm2 = a.m2;
         p.dz = a.dz; p.dy = a.dy; p.dx = a.dx; p.ny = a.ny; p.dz_on_load = 1;
         p.tw = a.tw;
+        p.perm4 = a.perm4;
         if (launch_fft2_pass<V>(p, a.L, a.sign, PM_ROW_LOAD, a.tag ? a.tag : "row_load", s)) return;
     }
+    HASTY_REQUIRE(a.perm4 == 0, "the class-major row layout needs a register FFT pass");
     const int64_t lines = a.nrows * a.nbatch;
     if (lines == 0) return;
     const size_t bytes = size_t(2) * a.nl * a.LP * sizeof(V);
@@ -172,6 +175,7 @@ void launch_col_pass(ColArgs<V> a, hipStream_t s) {
         p.scale = a.scale;
         p.psf = a.psf; p.psf_o = a.psf_o; p.psf_i = 1; p.psf_s = a.psf_s;
         p.tw = a.tw;
+        p.psf_q = a.psf_q;
         if (launch_fft2_pass<V>(p, a.L, a.sign, a.psf ? PM_COL_PSF : PM_COL, a.tag ? a.tag : "col_pass", s)) return;
     }
     const int64_t blocks = ceil_div(a.ni, a.tx) * int64_t(a.no) * a.nbatch;

@@ -34,6 +34,8 @@ int64_t next_smooth(int64_t n);
 // Device twiddle table tw[k] = exp(-2 pi i k / L), k in [0, L), cached per
 // (device, L, precision); lives until hasty_release_caches / process end.
 const void* twiddles(int L, bool f64);
+// HASTY_FFT_STOCKHAM set: the LDS Stockham kernels replace the register passes
+bool fft_force_stockham();
 
 // --------------------------------------------------------------------------
 // Radix butterflies: X[q] = sum_r v[r] exp(sign * 2 pi i r q / R)
@@ -197,6 +199,7 @@ template <class V> struct RowLoadArgs {
     int ny;                                  // row r -> (z = r / ny, y = r % ny)
     FFTPlan1D plan; const V* tw;
     const char* tag;                         // launch name for profiling (host only)
+    int perm4 = 0;                           // class-major store (register passes only)
 };
 
 template <class V> struct RowStoreArgs {
@@ -227,6 +230,7 @@ template <class V> struct ColArgs {
     decltype(V::x) scale;
     FFTPlan1D plan; const V* tw;
     const char* tag;
+    int psf_q = 0;                           // class-major kernel columns: line i reads 4 (i % psf_q) + i / psf_q
 };
 
 constexpr int kRowAccMax = 16;
@@ -366,11 +370,13 @@ __global__ void __launch_bounds__(kFFTThreads) col_pass_kernel(ColArgs<V> a) {
     V* res = lds_fft(buf0, buf1, tx, a.LP, a.plan, a.sign, a.tw);
     int sign_out = a.sign;
     if (a.psf) {
-        const V* k = a.psf + int64_t(o) * a.psf_o + i0;
+        const V* k = a.psf + int64_t(o) * a.psf_o;
         for (int t = threadIdx.x; t < tx * a.L; t += blockDim.x) {
             const int c = t % tx;
             const int p = t / tx;
-            res[c * a.LP + p] = cmul(res[c * a.LP + p], k[int64_t(p) * a.psf_s + c]);
+            const int i = i0 + c;
+            const int kc = a.psf_q ? 4 * (i % a.psf_q) + i / a.psf_q : i;   // class-major columns
+            res[c * a.LP + p] = cmul(res[c * a.LP + p], k[int64_t(p) * a.psf_s + kc]);
         }
         __syncthreads();
         V* other = (res == buf0) ? buf1 : buf0;

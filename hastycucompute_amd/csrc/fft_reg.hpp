@@ -188,6 +188,11 @@ template <class V> struct PassArgs {
     // the unshifted transform, twist_in input p by i^(p mod 4), each taken
     // through the pass's sign convention (see the twiddle fill)
     int twist_out, twist_in;
+    // class-major x layout of the fused Toeplitz path (both 0 elsewhere):
+    // perm4 (row-load passes with out_len = L, L % 4 == 0) stores output k
+    // at (k mod 4) (L/4) + k/4; psf_q = ni / 4 (PSF passes) reads, for line
+    // i, kernel line 4 (i mod psf_q) + i / psf_q, the frequency stored there
+    int perm4, psf_q;
 };
 
 template <int L1, int L2> struct RegShape {
@@ -351,7 +356,8 @@ __global__ void __launch_bounds__(512) fft2_pass_kernel(PassArgs<V> a) {
     if constexpr (PSF) {
         if (t < L1) {
             const char* kb = reinterpret_cast<const char*>(a.psf + int64_t(o) * a.psf_o);
-            const uint32_t ko = uint32_t(ic) * a.psf_ib + uint32_t(t) * a.psf_sb;
+            const int kcol = a.psf_q ? 4 * (ic % a.psf_q) + ic / a.psf_q : ic;
+            const uint32_t ko = uint32_t(kcol) * a.psf_ib + uint32_t(t) * a.psf_sb;
 #pragma unroll
             for (int k2 = 0; k2 < L2; ++k2) kv[k2] = load_at<V>(kb, ko + uint32_t(L1 * k2) * a.psf_sb);
         }
@@ -595,8 +601,33 @@ __global__ void __launch_bounds__(512) fft2_pass_kernel(PassArgs<V> a) {
                 }
             }
         } else {
+            bool staged = false;
+            if constexpr (MODE == PM_ROW_LOAD && L % 4 == 0) {
+                if (a.perm4) {
+                    // class-major store (full lines, no shift): stage the line
+                    // in natural order, then each thread stores consecutive
+                    // positions q = r L/4 + m, which hold frequency 4 m + r
+                    staged = true;
+                    __syncthreads();
+                    if (t < NOWNERS) {
+#pragma unroll
+                        for (int k = 0; k < NOWN; ++k) xl[t + OSTRIDE * k] = v[k];
+                    }
+                    __syncthreads();
+                    if (line_ok) {
+                        char* dst = reinterpret_cast<char*>(a.out + b * a.out_b + int64_t(o) * a.out_o);
+                        const uint32_t lo = uint32_t(icl) * a.out_ib;
+#pragma unroll
+                        for (int u = 0; u < L / T; ++u) {
+                            const int q = t + T * u;
+                            const int f = 4 * (q % (L / 4)) + q / (L / 4);
+                            store_at<V>(dst, lo + uint32_t(q) * out_sb, cscale(xl[f], a.scale));
+                        }
+                    }
+                }
+            }
             // owner t holds positions t + OSTRIDE k, k < NOWN
-            if (t < NOWNERS && line_ok) {
+            if (!staged && t < NOWNERS && line_ok) {
                 char* dst = reinterpret_cast<char*>(a.out + b * a.out_b + int64_t(o) * a.out_o);
                 const uint32_t lo = uint32_t(icl) * a.out_ib;
 #pragma unroll

@@ -10,10 +10,13 @@
 // The kernel K is read once per (ky, kx) column tile per batch element and
 // the (2n)^d grid never exists: P1/P2 skip the all-zero lines (pruning) and
 // P4/P5 only produce what the crop keeps.
+#include <algorithm>
+#include <cstdlib>
 #include <memory>
 #include <vector>
 
 #include "nufft.hpp"
+#include "toeplitz_fused.hpp"
 
 namespace hasty {
 
@@ -40,6 +43,24 @@ static ToeplitzDims toeplitz_dims(int dim, const int64_t* n) {
     return d;
 }
 
+// The fused last pass (toeplitz_fused.hip) replaces P4 + P5 for coil-summed
+// 3-D applies on 128 x 128 planes; P1 then stores x class-major and P3 reads
+// the kernel through the same permutation.  HASTY_TOEP_FUSED=0 turns it off;
+// HASTY_TOEP_GROUPS sets its coil groups (default: about 256 blocks).
+static bool toep_fused_enabled() {
+    static const bool v = [] {
+        const char* e = std::getenv("HASTY_TOEP_FUSED");
+        return !(e && e[0] == '0');
+    }();
+    return v;
+}
+static int toep_fused_groups(int64_t nz, int64_t B) {
+    const char* e = std::getenv("HASTY_TOEP_GROUPS");   // read per apply (tests set it)
+    const int env = e ? std::atoi(e) : 0;
+    int64_t g = env > 0 ? env : std::max<int64_t>(1, (256 + nz - 1) / nz);
+    return int(std::max<int64_t>(1, std::min<int64_t>(g, B)));
+}
+
 // Core Toeplitz apply on one batch (see file comment).
 static void toeplitz_core(const ToeplitzDims& d, int64_t B, const float2* input, float2* output,
                           const float2* kernel, float2* bufA, float2* bufB, DiagOp<float2> in1,
@@ -47,6 +68,8 @@ static void toeplitz_core(const ToeplitzDims& d, int64_t B, const float2* input,
                           DiagOp<float2> io, int accumulate, int sum_batch, hipStream_t s) {
     const int64_t nz = d.n[0], ny = d.n[1], nx = d.n[2];
     const int64_t Lz = d.dim == 3 ? 2 * nz : 1, Ly = 2 * ny, Lx = 2 * nx;
+    const bool fused = sum_batch && d.dim == 3 && B >= 1 && toep_crop_sum_supported(ny, nx) &&
+                       toep_fused_enabled() && !fft_force_stockham();
     // P1
     RowLoadArgs<float2> r{};
     r.in = input; r.out = bufA;
@@ -60,6 +83,7 @@ static void toeplitz_core(const ToeplitzDims& d, int64_t B, const float2* input,
     r.plan = make_fft_plan(int(Lx));
     r.tw = reinterpret_cast<const float2*>(twiddles(int(Lx), false));
     r.tag = "toep_P1_x_pad_fft";
+    r.perm4 = fused ? 1 : 0;
     launch_row_load<float2>(r, s);
     if (d.dim == 3) {
         // P2
@@ -77,8 +101,23 @@ static void toeplitz_core(const ToeplitzDims& d, int64_t B, const float2* input,
         z.out = bufB; z.out_s = Ly * Lx; z.out_o = Lx; z.out_b = nz * Ly * Lx;
         z.out_len = int(nz); z.out_shift = int(nz - 1); z.sign = -1;
         z.psf = kernel; z.psf_o = Lx; z.psf_s = Ly * Lx;
+        z.psf_q = fused ? int(Lx / 4) : 0;
     z.tag = "toep_P3_z_psf";
         launch_col_pass<float2>(z, s);
+        if (fused) {
+            // P4 + P5 in one pass; bufA (the P1 output, consumed by P2) holds the partial sums
+            ToepCropSumArgs f{};
+            f.grid = bufB; f.grid_b = d.nB();
+            f.out = output;
+            f.partial = bufA; f.partial_elems = B * d.nA();
+            f.m1 = out1; f.m2 = out2; f.io = io;
+            f.scale = float(1.0 / double(d.nK()));
+            f.nbatch = int(B); f.nz = int(nz);
+            f.groups = toep_fused_groups(nz, B);
+            f.accumulate = accumulate;
+            launch_toep_crop_sum(f, s);
+            return;
+        }
         // P4
         ColArgs<float2> y = make_col_args<float2>(int(Ly));
         y.in = bufB; y.ni = int(Lx); y.no = int(nz); y.nbatch = int(B);

@@ -1,0 +1,271 @@
+// Fused last pass of the Toeplitz-embedded SENSE normal operator on gfx950.
+//
+// Replaces the y inverse pass (P4) and the x inverse pass with the coil sum
+// (P5) for planes of 128 x 128 (2n = 256 per axis), so the [B, nz, ny, 2nx]
+// intermediate between them never goes through HBM.
+//
+// The x axis of the grid is class-major (the first pass stores frequency
+// k = 4m + r at position r L/4 + m), so class r of a row is 64 contiguous
+// values.  The 256-point inverse DFT along x splits by residue classes:
+//     x[n] = sum_r e^{+2 pi i r n / 256} inner_r(n mod 64),
+//     inner_r(n') = sum_m X[4m + r] e^{+2 pi i m n' / 64},
+// so each class costs one 64-point DFT and the four classes are summed with
+// a twiddle.  One block owns a plane z and a group of coils b; per (b, r):
+//   A  the 64 class columns of the (2ny, L/4) slab are read (512 B per row),
+//      inverse-transformed along y (two-level 16 x 16, one LDS exchange) and
+//      the crop [ny-1, 2ny-1) is written to LDS rows;
+//   B  each row's 64-point inverse DFT (8 x 8, one exchange) gives inner_r,
+//      accumulated into the crop [nx-1, 2nx-1): n = n' + 128 for every n' and
+//      n = n' + 192 for n' < 63, n = 127 for n' = 63 (16 outputs a thread).
+// After a coil's four classes the thread scales its 16 pixels, applies the
+// output diagonals and adds them to the coil sum.  Traffic per apply: the
+// (2ny, 2nx) planes read once, the maps once, the image (or partial sums)
+// written once.
+#include "toeplitz_fused.hpp"
+
+#include <cstdlib>
+
+#include "fft_reg.hpp"
+
+namespace hasty {
+
+namespace {
+constexpr int kNY = 128, kNX = 128;         // plane shape (ny, nx)
+constexpr int kLY = 2 * kNY, kLX = 2 * kNX;  // 256, 256
+constexpr int kM = kLX / 4;                 // 64 columns per class
+#ifndef HASTY_TOEP_FUSED_H
+#define HASTY_TOEP_FUSED_H 2
+#endif
+constexpr int kH = HASTY_TOEP_FUSED_H;       // lines per thread in both phases
+constexpr int kThreads = 1024 / kH;
+constexpr int kRS = kM + 8;                 // LDS row stride of the class rows
+constexpr int kE = 16 * 16 * kM;            // y exchange, [k1][p2][m]
+constexpr int kR = kNY * kRS;               // one set of class rows
+constexpr int kLdsElems = (kE > 2 * kR ? kE : 2 * kR) + kLX;
+static_assert(kH * kThreads == 16 * kM && kH * kThreads == 8 * kNY, "thread layout");
+}  // namespace
+
+// output pixel of slot i of lane s: slot k2 < 8 -> x = s + 8 k2 + 1, slot
+// 8 + k2 -> x = s + 8 k2 + 65 (n' = s + 8 k2 < 63) or x = 0 (n' = 63)
+__device__ inline int crop_px(int s, int i) {
+    const int np = s + 8 * (i & 7);
+    return i < 8 ? np + 1 : (np == 63 ? 0 : np + 65);
+}
+
+// v (.) d at element (b, row, x) for the 16 pixels of lane s
+__device__ inline void crop_sum_diag(float2* v, const DiagOp<float2>& d, int64_t b, int64_t row, int s) {
+    if (!d.ptr) return;
+    const float2* base = d.ptr + b * d.sb + row * d.sr;
+    const float sgn = d.type == HASTY_MULT_CONJ ? -1.f : 1.f;
+    // two rounds of 8 loads (the next columns are in flight in registers)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        float2 w[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) w[i] = base[crop_px(s, 8 * h + i)];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[8 * h + i] = cmul_sgn(v[8 * h + i], w[i], sgn);
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 4 / kH)))
+toep_crop_sum_kernel(ToepCropSumArgs a, const float2* tw) {
+    using V = float2;
+    extern __shared__ unsigned char smem_raw[];
+    V* lds = reinterpret_cast<V*>(smem_raw);
+    V* E = lds;              // phase A exchange
+    V* R1 = lds;             // class rows (after the exchange is read)
+    V* R2 = lds + kR;        // phase B step-1 results
+    V* twl = lds + (kLdsElems - kLX);
+    const int tid = threadIdx.x;
+    for (int k = tid; k < kLX; k += kThreads) twl[k] = tw[k];
+
+    const int z = int(blockIdx.x % unsigned(a.nz));
+    const int g = int(blockIdx.x / unsigned(a.nz));
+    const int cpb = (a.nbatch + a.groups - 1) / a.groups;
+    const int b0 = g * cpb;
+    const int b1 = min(a.nbatch, b0 + cpb);
+
+    // phase A roles: column m; step-1 lines p2 = pa, pa + 8 and step-2 lines
+    // k1 = pa, pa + 8 (pa = wave)
+    const int m0 = tid & (kM - 1);
+    const int pa0 = tid >> 6;
+    // phase B roles: rows y = yb, yb + 64, lane s of the 8 x 8 DFT
+    const int yb0 = tid >> 3;
+    const int s0 = tid & 7;
+    V tot[kH][16];
+#pragma unroll
+    for (int h = 0; h < kH; ++h)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) tot[h][i] = make_float2(0.f, 0.f);
+    __syncthreads();
+
+    // (coil, class) iterations it = (b - b0) 4 + r; the first half of the loads
+    // of iteration it + 1 is issued once iteration it has moved its columns to
+    // LDS, so it is in flight during the row transforms and the coil epilogue
+    // (the second half waits for the next iteration: registers)
+    const int nit = 4 * max(0, b1 - b0);
+    V v[kH][16];
+    auto load_class = [&](int it, int h0, int h1) {
+        int m = m0, pa = pa0;
+        asm volatile("" : "+v"(m), "+v"(pa));
+        const int bb = b0 + (it >> 2), rr = it & 3;
+        const char* plane =
+            reinterpret_cast<const char*>(a.grid + int64_t(bb) * a.grid_b + int64_t(z) * (kLY * kLX));
+        const uint32_t off = uint32_t((pa * kLX + rr * kM + m) * int(sizeof(V)));
+#pragma unroll
+        for (int h = 0; h < kH; ++h)
+            if (h >= h0 && h < h1)
+#pragma unroll
+                for (int p1 = 0; p1 < 16; ++p1)
+                    v[h][p1] =
+                        load_at<V, true>(plane, off + uint32_t((16 * p1 + (16 / kH) * h) * kLX * int(sizeof(V))));
+    };
+    constexpr int kPre = kH > 1 ? kH / 2 : kH;   // lines per thread prefetched a phase early
+    if (nit > 0) load_class(0, 0, kPre);
+    V acc[kH][16];
+#pragma unroll 1
+    for (int it = 0; it < nit; ++it) {
+        const int b = b0 + (it >> 2), r = it & 3;
+        // per-iteration copies of the lane indices: otherwise every LDS and
+        // twiddle address derived from them is hoisted out of the loop and
+        // stays live in registers
+        int m = m0, pa = pa0, yb = yb0, s = s0;
+        asm volatile("" : "+v"(m), "+v"(pa), "+v"(yb), "+v"(s));
+        if (r == 0) {
+#pragma unroll
+            for (int h = 0; h < kH; ++h)
+#pragma unroll
+                for (int i = 0; i < 16; ++i) acc[h][i] = make_float2(0.f, 0.f);
+        }
+        if (kPre < kH) load_class(it, kPre, kH);
+        // ---- A: y inverse DFT of the class columns ----
+#pragma unroll
+        for (int h = 0; h < kH; ++h) {
+            const int p2 = pa + (16 / kH) * h;
+            RegDFT<16, +1, V>::run(v[h]);
+#pragma unroll
+            for (int k1 = 1; k1 < 16; ++k1) v[h][k1] = twiddle_mul<V, +1>(v[h][k1], twl, (p2 * k1) & (kLX - 1));
+        }
+        __syncthreads();   // the previous iteration's phase B reads of R1 / R2 are done
+#pragma unroll
+        for (int h = 0; h < kH; ++h)
+#pragma unroll
+            for (int k1 = 0; k1 < 16; ++k1) E[(k1 * 16 + pa + (16 / kH) * h) * kM + m] = v[h][k1];
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < kH; ++h) {
+            const int k1 = pa + (16 / kH) * h;
+#pragma unroll
+            for (int p2 = 0; p2 < 16; ++p2) v[h][p2] = E[(k1 * 16 + p2) * kM + m];
+            RegDFT<16, +1, V>::run(v[h]);   // v[h][k2] = X[k1 + 16 k2]
+        }
+        __syncthreads();   // E is read; R1 overwrites it
+#pragma unroll
+        for (int h = 0; h < kH; ++h) {
+#pragma unroll
+            for (int k2 = 0; k2 < 16; ++k2) {
+                const int j = pa + (16 / kH) * h + 16 * k2;
+                if (j >= kNY - 1 && j < 2 * kNY - 1) R1[(j - (kNY - 1)) * kRS + m] = v[h][k2];
+            }
+        }
+        if (it + 1 < nit) load_class(it + 1, 0, kPre);
+        __syncthreads();
+        // ---- B: 64-point inverse DFT of each class row, 8 x 8 ----
+        V w[kH][8];
+#pragma unroll
+        for (int h = 0; h < kH; ++h) {
+            const int y = yb + (128 / kH) * h;
+#pragma unroll
+            for (int p1 = 0; p1 < 8; ++p1) w[h][p1] = R1[y * kRS + 8 * p1 + s];
+            RegDFT<8, +1, V>::run(w[h]);
+#pragma unroll
+            for (int k1 = 1; k1 < 8; ++k1) w[h][k1] = twiddle_mul<V, +1>(w[h][k1], twl, (4 * s * k1) & (kLX - 1));
+#pragma unroll
+            for (int k1 = 0; k1 < 8; ++k1) R2[y * kRS + s * 8 + k1] = w[h][k1];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < kH; ++h) {
+            const int y = yb + (128 / kH) * h;
+#pragma unroll
+            for (int p2 = 0; p2 < 8; ++p2) w[h][p2] = R2[y * kRS + p2 * 8 + s];
+            RegDFT<8, +1, V>::run(w[h]);   // w[h][k2] = inner_r(s + 8 k2)
+            // combine: acc += e^{+2 pi i r n / 256} inner_r(n mod 64)
+#pragma unroll
+            for (int k2 = 0; k2 < 8; ++k2) {
+                const int np = s + 8 * k2;
+                const int na = np + 128, nb = np == 63 ? 127 : np + 192;
+                acc[h][k2] = cadd(acc[h][k2], twiddle_mul<V, +1>(w[h][k2], twl, (r * na) & (kLX - 1)));
+                acc[h][8 + k2] = cadd(acc[h][8 + k2], twiddle_mul<V, +1>(w[h][k2], twl, (r * nb) & (kLX - 1)));
+            }
+        }
+        if (r == 3) {
+            // ---- coil epilogue: scale, output diagonals, coil sum ----
+#pragma unroll
+            for (int h = 0; h < kH; ++h) {
+                const int64_t row = int64_t(z) * kNY + yb + (128 / kH) * h;
+#pragma unroll
+                for (int i = 0; i < 16; ++i) acc[h][i] = cscale(acc[h][i], a.scale);
+                crop_sum_diag(acc[h], a.m1, b, row, s);
+                crop_sum_diag(acc[h], a.m2, b, row, s);
+                crop_sum_diag(acc[h], a.io, b, row, s);
+#pragma unroll
+                for (int i = 0; i < 16; ++i) tot[h][i] = cadd(tot[h][i], acc[h][i]);
+            }
+        }
+    }
+    const int yb = yb0, s = s0;
+#pragma unroll
+    for (int h = 0; h < kH; ++h) {
+        const int64_t row = int64_t(z) * kNY + yb + (128 / kH) * h;
+        if (a.groups > 1) {
+            V* part = a.partial + (int64_t(g) * a.nz * kNY + row) * kNX;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) part[crop_px(s, i)] = tot[h][i];
+        } else {
+            V* o = a.out + row * kNX;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int x = crop_px(s, i);
+                o[x] = a.accumulate ? cadd(tot[h][i], o[x]) : tot[h][i];
+            }
+        }
+    }
+}
+
+bool toep_crop_sum_supported(int64_t ny, int64_t nx) { return ny == kNY && nx == kNX; }
+
+void launch_toep_crop_sum(const ToepCropSumArgs& a, hipStream_t s) {
+    HASTY_REQUIRE(a.nz >= 1 && a.nbatch >= 1, "fused crop pass: empty batch or plane count");
+    HASTY_REQUIRE(a.groups >= 1 && a.groups <= a.nbatch, "fused crop pass: invalid coil group count");
+    HASTY_REQUIRE(a.grid_b >= int64_t(a.nz) * kLY * kLX, "fused crop pass: grid stride too small");
+    const int64_t nimg = int64_t(a.nz) * kNY * kNX;
+    if (a.groups > 1)
+        HASTY_REQUIRE(a.partial && a.partial_elems >= int64_t(a.groups) * nimg,
+                      "fused crop pass: partial-sum scratch too small");
+    const size_t bytes = size_t(kLdsElems) * sizeof(float2);
+    static const bool lds_ok = [&] {
+        HASTY_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(toep_crop_sum_kernel),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)));
+        return true;
+    }();
+    (void)lds_ok;
+    const float2* tw = reinterpret_cast<const float2*>(twiddles(kLX, false));
+    {
+        HASTY_PROF("toep_F45_yx_ifft_crop_sum", s);
+        hipLaunchKernelGGL(toep_crop_sum_kernel, dim3(unsigned(int64_t(a.nz) * a.groups)), dim3(kThreads), bytes,
+                           s, a, tw);
+        HASTY_CHECK_HIP(hipGetLastError());
+    }
+    if (a.groups > 1) {
+        HASTY_PROF("toep_F45 sum_partials", s);
+        const int64_t n = nimg;
+        hipLaunchKernelGGL(sum_partials_kernel<float2>, dim3(unsigned(std::min<int64_t>(ceil_div(n, 256), 4096))),
+                           dim3(256), 0, s, static_cast<const float2*>(a.partial), a.out, int64_t(a.nz) * kNY, kNX,
+                           int64_t(kNX), a.groups, a.accumulate);
+        HASTY_CHECK_HIP(hipGetLastError());
+    }
+}
+
+}  // namespace hasty

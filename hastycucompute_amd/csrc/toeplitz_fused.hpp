@@ -1,0 +1,26 @@
+// Fused last pass of the Toeplitz-embedded SENSE normal operator (3-D,
+// 128 x 128 planes): y inverse FFT + crop, x inverse FFT + crop, scale,
+// output diagonals and the coil sum in one kernel.  Host interface.
+#pragma once
+
+#include "fft.hpp"
+
+namespace hasty {
+
+struct ToepCropSumArgs {
+    const float2* grid;      // [B, nz, 2ny, 2nx], x positions class-major
+    int64_t grid_b;          // elements per batch element
+    float2* out;             // [nz, ny, nx]
+    float2* partial;         // [groups, nz, ny, nx] scratch when groups > 1
+    int64_t partial_elems;
+    DiagOp<float2> m1, m2, io;   // element (b, row, x) at ptr[b*sb + row*sr + x], row = z*ny + y
+    float scale;
+    int nbatch, nz, groups, accumulate;
+};
+
+// plane shapes the fused pass is built for (ny == nx == 128)
+bool toep_crop_sum_supported(int64_t ny, int64_t nx);
+// groups: coil groups per plane (partial sums added by a second kernel)
+void launch_toep_crop_sum(const ToepCropSumArgs& a, hipStream_t s);
+
+}  // namespace hasty
