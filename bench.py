@@ -116,6 +116,7 @@ def cpu_baseline(N, C, smaps_np, x_np, kernel_np, sample_coils):
 # calibration kernel: torch.neg over 1 GiB reads and writes exactly 2^30 B.
 PMC_KERNELS = {
     "toep_P3_z_psf": "fft2_pass_kernel<HIP_vector_type<float, 2u>, 16, 16, -1, 1, 3>",
+    "toep_F45_yx_ifft_crop_sum": "toep_crop_sum_kernel",
     "nufft_spread": "spread_kernel_w6<HIP_vector_type<float, 2u>, 32>",
     "nufft_interp": "interp_pair_kernel<6, 16, 16, 16>",
 }

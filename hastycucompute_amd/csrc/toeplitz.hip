@@ -44,9 +44,10 @@ static ToeplitzDims toeplitz_dims(int dim, const int64_t* n) {
 }
 
 // The fused last pass (toeplitz_fused.hip) replaces P4 + P5 for coil-summed
-// 3-D applies on 128 x 128 planes; P1 then stores x class-major and P3 reads
-// the kernel through the same permutation.  HASTY_TOEP_FUSED=0 turns it off;
-// HASTY_TOEP_GROUPS sets its coil groups (default: about 256 blocks).
+// 3-D applies of at least two coils on 128 x 128 planes; P1 then stores x
+// class-major and P3 reads the kernel through the same permutation.
+// HASTY_TOEP_FUSED=0 turns it off; HASTY_TOEP_GROUPS sets its coil groups
+// (default: about 256 blocks).
 static bool toep_fused_enabled() {
     static const bool v = [] {
         const char* e = std::getenv("HASTY_TOEP_FUSED");
@@ -68,7 +69,9 @@ static void toeplitz_core(const ToeplitzDims& d, int64_t B, const float2* input,
                           DiagOp<float2> io, int accumulate, int sum_batch, hipStream_t s) {
     const int64_t nz = d.n[0], ny = d.n[1], nx = d.n[2];
     const int64_t Lz = d.dim == 3 ? 2 * nz : 1, Ly = 2 * ny, Lx = 2 * nx;
-    const bool fused = sum_batch && d.dim == 3 && B >= 1 && toep_crop_sum_supported(ny, nx) &&
+    // (one coil per launch: a plane per block leaves half the CUs idle at
+    // nz = 128, and P4 + P5 are faster there)
+    const bool fused = sum_batch && d.dim == 3 && B >= 2 && toep_crop_sum_supported(ny, nx) &&
                        toep_fused_enabled() && !fft_force_stockham();
     // P1
     RowLoadArgs<float2> r{};
