@@ -39,6 +39,9 @@ constexpr int kM = kLX / 4;                 // 64 columns per class
 constexpr int kH = HASTY_TOEP_FUSED_H;       // lines per thread in both phases
 constexpr int kThreads = 1024 / kH;
 constexpr int kRS = kM + 8;                 // LDS row stride of the class rows
+// (the 8 x 8 exchange of a row uses stride 9 within it: the step-1 writes and
+// step-2 reads of a wave then touch each bank twice, the minimum for 512 B)
+static_assert(7 * 9 + 7 < kRS, "8 x 8 exchange fits in a class row");
 constexpr int kE = 16 * 16 * kM;            // y exchange, [k1][p2][m]
 constexpr int kR = kNY * kRS;               // one set of class rows
 constexpr int kLdsElems = (kE > 2 * kR ? kE : 2 * kR) + kLX;
@@ -182,14 +185,14 @@ toep_crop_sum_kernel(ToepCropSumArgs a, const float2* tw) {
 #pragma unroll
             for (int k1 = 1; k1 < 8; ++k1) w[h][k1] = twiddle_mul<V, +1>(w[h][k1], twl, (4 * s * k1) & (kLX - 1));
 #pragma unroll
-            for (int k1 = 0; k1 < 8; ++k1) R2[y * kRS + s * 8 + k1] = w[h][k1];
+            for (int k1 = 0; k1 < 8; ++k1) R2[y * kRS + s * 9 + k1] = w[h][k1];
         }
         __syncthreads();
 #pragma unroll
         for (int h = 0; h < kH; ++h) {
             const int y = yb + (128 / kH) * h;
 #pragma unroll
-            for (int p2 = 0; p2 < 8; ++p2) w[h][p2] = R2[y * kRS + p2 * 8 + s];
+            for (int p2 = 0; p2 < 8; ++p2) w[h][p2] = R2[y * kRS + p2 * 9 + s];
             RegDFT<8, +1, V>::run(w[h]);   // w[h][k2] = inner_r(s + 8 k2)
             // combine: acc += e^{+2 pi i r n / 256} inner_r(n mod 64)
 #pragma unroll
