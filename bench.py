@@ -62,6 +62,8 @@ def algorithmic_bytes(N, C, M):
         "toep_P5_x_ifft_crop": b * (2 * C * I + C * I + I),    # rows, S in; image out
         # fused last pass (replaces P4 + P5 on 128 x 128 planes): [C,N,2N,2N] and S in, image out
         "toep_F45_yx_ifft_crop_sum": b * (4 * C * I + C * I + I),
+        # fused first pass (replaces P1 + P2): x and S in, [C,N,2N,2N] out
+        "toep_F12_xy_pad_fft": b * (I + C * I + 4 * C * I),
         # NUFFT kernels
         "nufft_interp": b * (C * G + C * M) + 4 * 3 * M + 4 * M * 7,   # grid, out, coords, sorted pts
         "nufft_spread": b * (C * G + C * M) + 4 * 3 * M + 4 * M * 7,
@@ -117,6 +119,7 @@ def cpu_baseline(N, C, smaps_np, x_np, kernel_np, sample_coils):
 PMC_KERNELS = {
     "toep_P3_z_psf": "fft2_pass_kernel<HIP_vector_type<float, 2u>, 16, 16, -1, 1, 3>",
     "toep_F45_yx_ifft_crop_sum": "toep_crop_sum_kernel",
+    "toep_F12_xy_pad_fft": "toep_pad_fft_kernel",
     "nufft_spread": "spread_kernel_w6<HIP_vector_type<float, 2u>, 32>",
     "nufft_interp": "interp_pair_kernel<6, 16, 16, 16>",
 }
@@ -179,7 +182,7 @@ _TRACE_CMD = ("cd /tmp && export TMPDIR=/tmp && cd $REPO && rocprofv3 --kernel-t
               "-d {d} -o run -- python -u bench.py --steps 3 --warmup 1 --no-cpu-baseline")
 # the launches this script times after its calibration kernel: (apply, type 2,
 # type 1) three times, each as (label, kernel-name substring)
-_TIMED_SEQ = ([("toep_P1_x_pad_fft", "<c64, 16, 16, -1, 2, 1>"), ("toep_P2_y_pad_fft", "<c64, 16, 16, -1, 0, 1>"),
+_TIMED_SEQ = ([("toep_F12_xy_pad_fft", "toep_pad_fft_kernel"),
                ("toep_P3_z_psf", "<c64, 16, 16, -1, 1, 3>"), ("toep_F45_yx_ifft_crop_sum", "toep_crop_sum_kernel"),
                ("toep_F45 sum_partials", "sum_partials")] +
               [("fwd_x_pad_fft", "<c64, 16, 16, -1, 2, 1>"), ("fwd_y_pad_fft", "<c64, 16, 16, -1, 0, 1>"),
@@ -237,8 +240,8 @@ def summarize_trace(trace_dir, workload, top=30):
             raise SystemExit(f"unexpected launch {name} where {tag} was expected")
         timed.setdefault((tag, name), []).append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6)
     out += ["", "## Launches timed by bench.py's roofline", "",
-            "The launches that follow the calibration kernel (`torch.neg` over 1 GiB): an A^H A apply (P1-P3, "
-            "the fused y/x inverse pass with the coil sum, and the coil-group partials), a type-2 and a type-1 transform at full size, three times over, which "
+            "The launches that follow the calibration kernel (`torch.neg` over 1 GiB): an A^H A apply (the fused x/y "
+            "pad-FFT pass, P3, the fused y/x inverse pass with the coil sum, and the coil-group partials), a type-2 and a type-1 transform at full size, three times over, which "
             "bench.py times with HIP events on the library stream. Several passes share a kernel instance, so "
             "each launch is labelled by its place in that sequence.", "",
             "| pass | kernel | launches | avg ms |", "|---|---|---:|---:|"]

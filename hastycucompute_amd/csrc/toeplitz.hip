@@ -43,15 +43,17 @@ static ToeplitzDims toeplitz_dims(int dim, const int64_t* n) {
     return d;
 }
 
-// The fused last pass (toeplitz_fused.hip) replaces P4 + P5 for coil-summed
-// 3-D applies of at least two coils on 128 x 128 planes; P1 then stores x
-// class-major and P3 reads the kernel through the same permutation.
-// HASTY_TOEP_FUSED=0 turns it off; HASTY_TOEP_GROUPS sets its coil groups
-// (default: about 256 blocks).
-static bool toep_fused_enabled() {
-    static const bool v = [] {
+// The fused passes (toeplitz_fused.hip) run coil-summed 3-D applies of at
+// least two coils on 128 x 128 planes: P1 + P2 in one kernel writing x
+// class-major, P3 reading the kernel through the same permutation, P4 + P5
+// in one kernel with the coil sum.  HASTY_TOEP_FUSED=0 turns them off, =2
+// keeps only the last one (P1 then stores class-major itself);
+// HASTY_TOEP_GROUPS sets the coil groups of the last one (default: about 256
+// blocks).
+static int toep_fused_mode() {
+    static const int v = [] {
         const char* e = std::getenv("HASTY_TOEP_FUSED");
-        return !(e && e[0] == '0');
+        return e ? (e[0] == '0' ? 0 : (e[0] == '2' ? 2 : 1)) : 1;
     }();
     return v;
 }
@@ -72,7 +74,16 @@ static void toeplitz_core(const ToeplitzDims& d, int64_t B, const float2* input,
     // (one coil per launch: a plane per block leaves half the CUs idle at
     // nz = 128, and P4 + P5 are faster there)
     const bool fused = sum_batch && d.dim == 3 && B >= 2 && toep_crop_sum_supported(ny, nx) &&
-                       toep_fused_enabled() && !fft_force_stockham();
+                       toep_fused_mode() != 0 && !fft_force_stockham();
+    const bool fused_first = fused && toep_fused_mode() == 1;
+    if (fused_first) {
+        ToepPadFftArgs f{};
+        f.in = input; f.in_b = d.nimg();
+        f.m1 = in1; f.m2 = in2;
+        f.grid = bufB; f.grid_b = d.nB();
+        f.nbatch = int(B); f.nz = int(nz);
+        launch_toep_pad_fft(f, s);
+    }
     // P1
     RowLoadArgs<float2> r{};
     r.in = input; r.out = bufA;
@@ -87,7 +98,7 @@ static void toeplitz_core(const ToeplitzDims& d, int64_t B, const float2* input,
     r.tw = reinterpret_cast<const float2*>(twiddles(int(Lx), false));
     r.tag = "toep_P1_x_pad_fft";
     r.perm4 = fused ? 1 : 0;
-    launch_row_load<float2>(r, s);
+    if (!fused_first) launch_row_load<float2>(r, s);
     if (d.dim == 3) {
         // P2
         ColArgs<float2> c = make_col_args<float2>(int(Ly));
@@ -96,7 +107,7 @@ static void toeplitz_core(const ToeplitzDims& d, int64_t B, const float2* input,
         c.out = bufB; c.out_s = Lx; c.out_o = Ly * Lx; c.out_b = nz * Ly * Lx;
         c.out_len = int(Ly); c.out_shift = 0; c.sign = -1;
     c.tag = "toep_P2_y_pad_fft";
-        launch_col_pass<float2>(c, s);
+        if (!fused_first) launch_col_pass<float2>(c, s);
         // P3 (in place on the first nz planes of bufB)
         ColArgs<float2> z = make_col_args<float2>(int(Lz));
         z.in = bufB; z.ni = int(Lx); z.no = int(Ly); z.nbatch = int(B);

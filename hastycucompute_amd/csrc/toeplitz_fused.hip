@@ -237,6 +237,171 @@ toep_crop_sum_kernel(ToepCropSumArgs a, const float2* tw) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// First two passes fused: x and y zero-padded forward DFTs, x class-major.
+// Per plane z and coil b the thread loads its two rows' 16 + 16 inputs once
+// (x positions n' and n' + 64, times the diagonals); per class r:
+//   B' each row's class: u[n'] = e^{-2 pi i r n' / 256} (x[n'] + (-i)^r
+//      x[n' + 64]), a 64-point DFT (8 x 8, one exchange) -> X[4m + r], rows to LDS;
+//   A' each class column: 128 inputs zero-padded to 256 (16 x 16 with half
+//      the first-step inputs zero), stored at (ky, r L/4 + m): 512 B per wave.
+// The next coil's inputs are loaded once the last class has folded them.
+// ---------------------------------------------------------------------------
+__device__ inline void pad_fft_diag(float2 (*xs)[16], const DiagOp<float2>& d, int64_t b, int z, int yb, int s) {
+    if (!d.ptr) return;
+    const float sgn = d.type == HASTY_MULT_CONJ ? -1.f : 1.f;
+#pragma unroll
+    for (int h = 0; h < kH; ++h) {
+        const int64_t row = int64_t(z) * kNY + yb + (128 / kH) * h;
+        const float2* base = d.ptr + b * d.sb + row * d.sr + s;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            float2 w[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) w[i] = base[8 * i + 64 * q];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) xs[h][8 * q + i] = cmul_sgn(xs[h][8 * q + i], w[i], sgn);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 4 / kH)))
+toep_pad_fft_kernel(ToepPadFftArgs a, const float2* tw, int groups) {
+    using V = float2;
+    extern __shared__ unsigned char smem_raw[];
+    V* lds = reinterpret_cast<V*>(smem_raw);
+    V* E = lds;              // column exchange
+    V* R1 = lds;             // class rows (read before the exchange is written)
+    V* R2 = lds + kR;        // row 8 x 8 exchange
+    V* twl = lds + (kLdsElems - kLX);
+    const int tid = threadIdx.x;
+    for (int k = tid; k < kLX; k += kThreads) twl[k] = tw[k];
+
+    const int z = int(blockIdx.x % unsigned(a.nz));
+    const int g = int(blockIdx.x / unsigned(a.nz));
+    const int cpb = (a.nbatch + groups - 1) / groups;
+    const int b0 = g * cpb;
+    const int b1 = min(a.nbatch, b0 + cpb);
+    const int m0 = tid & (kM - 1), pa0 = tid >> 6, yb0 = tid >> 3, s0 = tid & 7;
+
+    // xs[h][i]: row yb + (128 / kH) h, x = 8 i + s (i < 8) or 8 (i - 8) + 64 + s
+    V xs[kH][16];
+    auto load_in = [&](int b) {
+        int yb = yb0, s = s0;
+        asm volatile("" : "+v"(yb), "+v"(s));
+        const char* pl = reinterpret_cast<const char*>(a.in + int64_t(b) * a.in_b + int64_t(z) * (kNY * kNX));
+#pragma unroll
+        for (int h = 0; h < kH; ++h) {
+            const uint32_t off = uint32_t(((yb + (128 / kH) * h) * kNX + s) * int(sizeof(V)));
+#pragma unroll
+            for (int i = 0; i < 16; ++i)
+                xs[h][i] = load_at<V>(pl, off + uint32_t((8 * (i & 7) + 64 * (i >> 3)) * int(sizeof(V))));
+        }
+    };
+    if (b0 < b1) load_in(b0);
+    __syncthreads();
+
+#pragma unroll 1
+    for (int b = b0; b < b1; ++b) {
+        {
+            int yb = yb0, s = s0;
+            asm volatile("" : "+v"(yb), "+v"(s));
+            pad_fft_diag(xs, a.m1, b, z, yb, s);
+            pad_fft_diag(xs, a.m2, b, z, yb, s);
+        }
+        char* gp = reinterpret_cast<char*>(a.grid + int64_t(b) * a.grid_b + int64_t(z) * (kLY * kLX));
+#pragma unroll 1
+        for (int r = 0; r < 4; ++r) {
+            int m = m0, pa = pa0, yb = yb0, s = s0;
+            asm volatile("" : "+v"(m), "+v"(pa), "+v"(yb), "+v"(s));
+            // ---- B': fold, 64-point DFT of each row's class ----
+            V u[kH][8];
+#pragma unroll
+            for (int h = 0; h < kH; ++h) {
+#pragma unroll
+                for (int p1 = 0; p1 < 8; ++p1) {
+                    const int np = 8 * p1 + s;
+                    V t = cadd(xs[h][p1], quarter_turn(xs[h][8 + p1], 4 - r));
+                    u[h][p1] = r ? twiddle_mul<V, -1>(t, twl, (r * np) & (kLX - 1)) : t;
+                }
+            }
+            if (r == 3 && b + 1 < b1) load_in(b + 1);   // this coil's inputs are folded
+#pragma unroll
+            for (int h = 0; h < kH; ++h) {
+                RegDFT<8, -1, V>::run(u[h]);
+#pragma unroll
+                for (int k1 = 1; k1 < 8; ++k1) u[h][k1] = twiddle_mul<V, -1>(u[h][k1], twl, (4 * s * k1) & (kLX - 1));
+            }
+            __syncthreads();   // the previous class's exchange reads are done (E aliases R2)
+#pragma unroll
+            for (int h = 0; h < kH; ++h) {
+                const int y = yb + (128 / kH) * h;
+#pragma unroll
+                for (int k1 = 0; k1 < 8; ++k1) R2[y * kRS + s * 9 + k1] = u[h][k1];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int h = 0; h < kH; ++h) {
+                const int y = yb + (128 / kH) * h;
+#pragma unroll
+                for (int p2 = 0; p2 < 8; ++p2) u[h][p2] = R2[y * kRS + p2 * 9 + s];
+                RegDFT<8, -1, V>::run(u[h]);   // u[h][k2] = X_r[y][s + 8 k2]
+#pragma unroll
+                for (int k2 = 0; k2 < 8; ++k2) R1[y * kRS + s + 8 * k2] = u[h][k2];
+            }
+            __syncthreads();
+            // ---- A': y DFT of the class columns, input rows [0, ny) ----
+            V v[kH][16];
+#pragma unroll
+            for (int h = 0; h < kH; ++h) {
+                const int p2 = pa + (16 / kH) * h;
+#pragma unroll
+                for (int p1 = 0; p1 < 16; ++p1)
+                    v[h][p1] = p1 < 8 ? R1[(16 * p1 + p2) * kRS + m] : make_float2(0.f, 0.f);
+                RegDFT<16, -1, V>::run(v[h]);
+#pragma unroll
+                for (int k1 = 1; k1 < 16; ++k1) v[h][k1] = twiddle_mul<V, -1>(v[h][k1], twl, (p2 * k1) & (kLX - 1));
+            }
+            __syncthreads();   // R1 is read; the exchange overwrites it
+#pragma unroll
+            for (int h = 0; h < kH; ++h)
+#pragma unroll
+                for (int k1 = 0; k1 < 16; ++k1) E[(k1 * 16 + pa + (16 / kH) * h) * kM + m] = v[h][k1];
+            __syncthreads();
+#pragma unroll
+            for (int h = 0; h < kH; ++h) {
+                const int k1 = pa + (16 / kH) * h;
+#pragma unroll
+                for (int p2 = 0; p2 < 16; ++p2) v[h][p2] = E[(k1 * 16 + p2) * kM + m];
+                RegDFT<16, -1, V>::run(v[h]);   // v[h][k2] = column output ky = k1 + 16 k2
+                const uint32_t off = uint32_t((k1 * kLX + r * kM + m) * int(sizeof(V)));
+#pragma unroll
+                for (int k2 = 0; k2 < 16; ++k2)
+                    store_at<V, true>(gp, off + uint32_t(16 * k2 * kLX * int(sizeof(V))), v[h][k2]);
+            }
+        }
+    }
+}
+
+void launch_toep_pad_fft(const ToepPadFftArgs& a, hipStream_t s) {
+    HASTY_REQUIRE(a.nz >= 1 && a.nbatch >= 1, "fused pad pass: empty batch or plane count");
+    HASTY_REQUIRE(a.grid_b >= int64_t(a.nz) * kLY * kLX, "fused pad pass: grid stride too small");
+    HASTY_REQUIRE(a.in_b >= int64_t(a.nz) * kNY * kNX || a.nbatch == 1, "fused pad pass: input stride too small");
+    const size_t bytes = size_t(kLdsElems) * sizeof(float2);
+    static const bool lds_ok = [&] {
+        HASTY_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(toep_pad_fft_kernel),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)));
+        return true;
+    }();
+    (void)lds_ok;
+    const int groups = int(std::max<int64_t>(1, std::min<int64_t>((256 + a.nz - 1) / a.nz, a.nbatch)));
+    const float2* tw = reinterpret_cast<const float2*>(twiddles(kLX, false));
+    HASTY_PROF("toep_F12_xy_pad_fft", s);
+    hipLaunchKernelGGL(toep_pad_fft_kernel, dim3(unsigned(int64_t(a.nz) * groups)), dim3(kThreads), bytes, s, a, tw,
+                       groups);
+    HASTY_CHECK_HIP(hipGetLastError());
+}
+
 bool toep_crop_sum_supported(int64_t ny, int64_t nx) { return ny == kNY && nx == kNX; }
 
 void launch_toep_crop_sum(const ToepCropSumArgs& a, hipStream_t s) {

@@ -18,7 +18,19 @@ struct ToepCropSumArgs {
     int nbatch, nz, groups, accumulate;
 };
 
-// plane shapes the fused pass is built for (ny == nx == 128)
+// First two passes fused: x and y zero-padded forward FFTs of
+// in[b] (.) m1 (.) m2 into grid [B, nz, 2ny, 2nx] with x class-major.
+struct ToepPadFftArgs {
+    const float2* in;        // [B, nz, ny, nx]
+    int64_t in_b;
+    DiagOp<float2> m1, m2;   // element (b, row, x) at ptr[b*sb + row*sr + x]
+    float2* grid;            // [B, nz, 2ny, 2nx], x positions class-major
+    int64_t grid_b;
+    int nbatch, nz;
+};
+void launch_toep_pad_fft(const ToepPadFftArgs& a, hipStream_t s);
+
+// plane shapes the fused passes are built for (ny == nx == 128)
 bool toep_crop_sum_supported(int64_t ny, int64_t nx);
 // groups: coil groups per plane (partial sums added by a second kernel)
 void launch_toep_crop_sum(const ToepCropSumArgs& a, hipStream_t s);
