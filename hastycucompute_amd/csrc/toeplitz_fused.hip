@@ -36,7 +36,12 @@ constexpr int kM = kLX / 4;                 // 64 columns per class
 #ifndef HASTY_TOEP_FUSED_H
 #define HASTY_TOEP_FUSED_H 2
 #endif
-constexpr int kH = HASTY_TOEP_FUSED_H;       // lines per thread in both phases
+constexpr int kH = HASTY_TOEP_FUSED_H;       // lines per thread in both phases of the last pass
+// lines per thread of the first pass (one: 96 VGPRs at 1024 threads, 4 waves per SIMD)
+#ifndef HASTY_TOEP_F12_H
+#define HASTY_TOEP_F12_H 1
+#endif
+constexpr int kH12 = HASTY_TOEP_F12_H;
 constexpr int kThreads = 1024 / kH;
 constexpr int kRS = kM + 8;                 // LDS row stride of the class rows
 // (the 8 x 8 exchange of a row uses stride 9 within it: the step-1 writes and
@@ -247,12 +252,13 @@ toep_crop_sum_kernel(ToepCropSumArgs a, const float2* tw) {
 //      the first-step inputs zero), stored at (ky, r L/4 + m): 512 B per wave.
 // The next coil's inputs are loaded once the last class has folded them.
 // ---------------------------------------------------------------------------
+template <int H>
 __device__ inline void pad_fft_diag(float2 (*xs)[16], const DiagOp<float2>& d, int64_t b, int z, int yb, int s) {
     if (!d.ptr) return;
     const float sgn = d.type == HASTY_MULT_CONJ ? -1.f : 1.f;
 #pragma unroll
-    for (int h = 0; h < kH; ++h) {
-        const int64_t row = int64_t(z) * kNY + yb + (128 / kH) * h;
+    for (int h = 0; h < H; ++h) {
+        const int64_t row = int64_t(z) * kNY + yb + (128 / H) * h;
         const float2* base = d.ptr + b * d.sb + row * d.sr + s;
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
@@ -265,7 +271,8 @@ __device__ inline void pad_fft_diag(float2 (*xs)[16], const DiagOp<float2>& d, i
     }
 }
 
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 4 / kH)))
+template <int H>
+__global__ void __launch_bounds__(1024 / H) __attribute__((amdgpu_waves_per_eu(1, 4 / H)))
 toep_pad_fft_kernel(ToepPadFftArgs a, const float2* tw, int groups) {
     using V = float2;
     extern __shared__ unsigned char smem_raw[];
@@ -275,7 +282,7 @@ toep_pad_fft_kernel(ToepPadFftArgs a, const float2* tw, int groups) {
     V* R2 = lds + kR;        // row 8 x 8 exchange
     V* twl = lds + (kLdsElems - kLX);
     const int tid = threadIdx.x;
-    for (int k = tid; k < kLX; k += kThreads) twl[k] = tw[k];
+    for (int k = tid; k < kLX; k += (1024 / H)) twl[k] = tw[k];
 
     const int z = int(blockIdx.x % unsigned(a.nz));
     const int g = int(blockIdx.x / unsigned(a.nz));
@@ -284,15 +291,15 @@ toep_pad_fft_kernel(ToepPadFftArgs a, const float2* tw, int groups) {
     const int b1 = min(a.nbatch, b0 + cpb);
     const int m0 = tid & (kM - 1), pa0 = tid >> 6, yb0 = tid >> 3, s0 = tid & 7;
 
-    // xs[h][i]: row yb + (128 / kH) h, x = 8 i + s (i < 8) or 8 (i - 8) + 64 + s
-    V xs[kH][16];
+    // xs[h][i]: row yb + (128 / H) h, x = 8 i + s (i < 8) or 8 (i - 8) + 64 + s
+    V xs[H][16];
     auto load_in = [&](int b) {
         int yb = yb0, s = s0;
         asm volatile("" : "+v"(yb), "+v"(s));
         const char* pl = reinterpret_cast<const char*>(a.in + int64_t(b) * a.in_b + int64_t(z) * (kNY * kNX));
 #pragma unroll
-        for (int h = 0; h < kH; ++h) {
-            const uint32_t off = uint32_t(((yb + (128 / kH) * h) * kNX + s) * int(sizeof(V)));
+        for (int h = 0; h < H; ++h) {
+            const uint32_t off = uint32_t(((yb + (128 / H) * h) * kNX + s) * int(sizeof(V)));
 #pragma unroll
             for (int i = 0; i < 16; ++i)
                 xs[h][i] = load_at<V>(pl, off + uint32_t((8 * (i & 7) + 64 * (i >> 3)) * int(sizeof(V))));
@@ -306,8 +313,8 @@ toep_pad_fft_kernel(ToepPadFftArgs a, const float2* tw, int groups) {
         {
             int yb = yb0, s = s0;
             asm volatile("" : "+v"(yb), "+v"(s));
-            pad_fft_diag(xs, a.m1, b, z, yb, s);
-            pad_fft_diag(xs, a.m2, b, z, yb, s);
+            pad_fft_diag<H>(xs, a.m1, b, z, yb, s);
+            pad_fft_diag<H>(xs, a.m2, b, z, yb, s);
         }
         char* gp = reinterpret_cast<char*>(a.grid + int64_t(b) * a.grid_b + int64_t(z) * (kLY * kLX));
 #pragma unroll 1
@@ -315,9 +322,9 @@ toep_pad_fft_kernel(ToepPadFftArgs a, const float2* tw, int groups) {
             int m = m0, pa = pa0, yb = yb0, s = s0;
             asm volatile("" : "+v"(m), "+v"(pa), "+v"(yb), "+v"(s));
             // ---- B': fold, 64-point DFT of each row's class ----
-            V u[kH][8];
+            V u[H][8];
 #pragma unroll
-            for (int h = 0; h < kH; ++h) {
+            for (int h = 0; h < H; ++h) {
 #pragma unroll
                 for (int p1 = 0; p1 < 8; ++p1) {
                     const int np = 8 * p1 + s;
@@ -327,22 +334,22 @@ toep_pad_fft_kernel(ToepPadFftArgs a, const float2* tw, int groups) {
             }
             if (r == 3 && b + 1 < b1) load_in(b + 1);   // this coil's inputs are folded
 #pragma unroll
-            for (int h = 0; h < kH; ++h) {
+            for (int h = 0; h < H; ++h) {
                 RegDFT<8, -1, V>::run(u[h]);
 #pragma unroll
                 for (int k1 = 1; k1 < 8; ++k1) u[h][k1] = twiddle_mul<V, -1>(u[h][k1], twl, (4 * s * k1) & (kLX - 1));
             }
             __syncthreads();   // the previous class's exchange reads are done (E aliases R2)
 #pragma unroll
-            for (int h = 0; h < kH; ++h) {
-                const int y = yb + (128 / kH) * h;
+            for (int h = 0; h < H; ++h) {
+                const int y = yb + (128 / H) * h;
 #pragma unroll
                 for (int k1 = 0; k1 < 8; ++k1) R2[y * kRS + s * 9 + k1] = u[h][k1];
             }
             __syncthreads();
 #pragma unroll
-            for (int h = 0; h < kH; ++h) {
-                const int y = yb + (128 / kH) * h;
+            for (int h = 0; h < H; ++h) {
+                const int y = yb + (128 / H) * h;
 #pragma unroll
                 for (int p2 = 0; p2 < 8; ++p2) u[h][p2] = R2[y * kRS + p2 * 9 + s];
                 RegDFT<8, -1, V>::run(u[h]);   // u[h][k2] = X_r[y][s + 8 k2]
@@ -351,10 +358,10 @@ toep_pad_fft_kernel(ToepPadFftArgs a, const float2* tw, int groups) {
             }
             __syncthreads();
             // ---- A': y DFT of the class columns, input rows [0, ny) ----
-            V v[kH][16];
+            V v[H][16];
 #pragma unroll
-            for (int h = 0; h < kH; ++h) {
-                const int p2 = pa + (16 / kH) * h;
+            for (int h = 0; h < H; ++h) {
+                const int p2 = pa + (16 / H) * h;
 #pragma unroll
                 for (int p1 = 0; p1 < 16; ++p1)
                     v[h][p1] = p1 < 8 ? R1[(16 * p1 + p2) * kRS + m] : make_float2(0.f, 0.f);
@@ -364,13 +371,13 @@ toep_pad_fft_kernel(ToepPadFftArgs a, const float2* tw, int groups) {
             }
             __syncthreads();   // R1 is read; the exchange overwrites it
 #pragma unroll
-            for (int h = 0; h < kH; ++h)
+            for (int h = 0; h < H; ++h)
 #pragma unroll
-                for (int k1 = 0; k1 < 16; ++k1) E[(k1 * 16 + pa + (16 / kH) * h) * kM + m] = v[h][k1];
+                for (int k1 = 0; k1 < 16; ++k1) E[(k1 * 16 + pa + (16 / H) * h) * kM + m] = v[h][k1];
             __syncthreads();
 #pragma unroll
-            for (int h = 0; h < kH; ++h) {
-                const int k1 = pa + (16 / kH) * h;
+            for (int h = 0; h < H; ++h) {
+                const int k1 = pa + (16 / H) * h;
 #pragma unroll
                 for (int p2 = 0; p2 < 16; ++p2) v[h][p2] = E[(k1 * 16 + p2) * kM + m];
                 RegDFT<16, -1, V>::run(v[h]);   // v[h][k2] = column output ky = k1 + 16 k2
@@ -389,7 +396,7 @@ void launch_toep_pad_fft(const ToepPadFftArgs& a, hipStream_t s) {
     HASTY_REQUIRE(a.in_b >= int64_t(a.nz) * kNY * kNX || a.nbatch == 1, "fused pad pass: input stride too small");
     const size_t bytes = size_t(kLdsElems) * sizeof(float2);
     static const bool lds_ok = [&] {
-        HASTY_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(toep_pad_fft_kernel),
+        HASTY_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(toep_pad_fft_kernel<kH12>),
                                             hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)));
         return true;
     }();
@@ -397,8 +404,8 @@ void launch_toep_pad_fft(const ToepPadFftArgs& a, hipStream_t s) {
     const int groups = int(std::max<int64_t>(1, std::min<int64_t>((256 + a.nz - 1) / a.nz, a.nbatch)));
     const float2* tw = reinterpret_cast<const float2*>(twiddles(kLX, false));
     HASTY_PROF("toep_F12_xy_pad_fft", s);
-    hipLaunchKernelGGL(toep_pad_fft_kernel, dim3(unsigned(int64_t(a.nz) * groups)), dim3(kThreads), bytes, s, a, tw,
-                       groups);
+    hipLaunchKernelGGL(toep_pad_fft_kernel<kH12>, dim3(unsigned(int64_t(a.nz) * groups)), dim3(1024 / kH12), bytes, s,
+                       a, tw, groups);
     HASTY_CHECK_HIP(hipGetLastError());
 }
 
