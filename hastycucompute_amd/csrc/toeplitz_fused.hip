@@ -1,29 +1,36 @@
-// Fused last pass of the Toeplitz-embedded SENSE normal operator on gfx950.
+// Fused Toeplitz passes of the SENSE normal operator on gfx950, for planes of
+// 128 x 128 (2n = 256 per axis):
+//   toep_pad_fft_kernel   x and y zero-padded forward FFTs (replaces P1 + P2)
+//   toep_crop_sum_kernel  y and x inverse FFTs, crops, output diagonals and
+//                         the coil sum (replaces P4 + P5)
+// so neither [B, nz, ny, 2nx] intermediate goes through HBM.  Between them
+// the z pass (P3) runs unchanged on the grid, reading the kernel through the
+// permutation below.
 //
-// Replaces the y inverse pass (P4) and the x inverse pass with the coil sum
-// (P5) for planes of 128 x 128 (2n = 256 per axis), so the [B, nz, ny, 2nx]
-// intermediate between them never goes through HBM.
-//
-// The x axis of the grid is class-major (the first pass stores frequency
-// k = 4m + r at position r L/4 + m), so class r of a row is 64 contiguous
-// values.  The 256-point inverse DFT along x splits by residue classes:
-//     x[n] = sum_r e^{+2 pi i r n / 256} inner_r(n mod 64),
+// The x axis of the grid is class-major: frequency k = 4m + r sits at
+// position r L/4 + m, so class r of a row is 64 contiguous values.  The
+// 256-point DFT along x splits by residue classes:
+//     X[4m + r] = sum_{n'<64} e^{-2 pi i r n'/256} (x[n'] + (-i)^r x[n'+64]) e^{-2 pi i m n'/64}
+//     x[n]      = sum_r e^{+2 pi i r n / 256} inner_r(n mod 64),
 //     inner_r(n') = sum_m X[4m + r] e^{+2 pi i m n' / 64},
-// so each class costs one 64-point DFT and the four classes are summed with
-// a twiddle.  One block owns a plane z and a group of coils b; per (b, r):
-//   A  the 64 class columns of the (2ny, L/4) slab are read (512 B per row),
-//      inverse-transformed along y (two-level 16 x 16, one LDS exchange) and
-//      the crop [ny-1, 2ny-1) is written to LDS rows;
-//   B  each row's 64-point inverse DFT (8 x 8, one exchange) gives inner_r,
-//      accumulated into the crop [nx-1, 2nx-1): n = n' + 128 for every n' and
-//      n = n' + 192 for n' < 63, n = 127 for n' = 63 (16 outputs a thread).
+// so each class costs one 64-point DFT (the same work as one 256-point DFT
+// over the four classes).  One block owns a plane z and a group of coils b;
+// per (b, r) the inverse pass:
+//   A  reads the 64 class columns of the (2ny, L/4) slab (512 B per row),
+//      inverse-transforms them along y (two-level 16 x 16, one LDS exchange)
+//      and writes the crop [ny-1, 2ny-1) to LDS rows;
+//   B  runs each row's 64-point inverse DFT (8 x 8, one exchange), giving
+//      inner_r, and accumulates it into the crop [nx-1, 2nx-1): n = n' + 128
+//      for every n', and n = n' + 192 for n' < 63 or n = 127 for n' = 63
+//      (16 outputs a thread).
 // After a coil's four classes the thread scales its 16 pixels, applies the
-// output diagonals and adds them to the coil sum.  Traffic per apply: the
-// (2ny, 2nx) planes read once, the maps once, the image (or partial sums)
-// written once.
+// output diagonals and adds them to the coil sum.  The forward pass runs the
+// same two phases in the other order (rows, then columns).  Traffic per
+// apply: the (2ny, 2nx) planes written once and read once, the maps read
+// twice, the image (or partial sums) written once.
 #include "toeplitz_fused.hpp"
 
-#include <cstdlib>
+#include <algorithm>
 
 #include "fft_reg.hpp"
 
