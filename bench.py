@@ -260,9 +260,22 @@ _SQ_KERNELS = (("spread_kernel_w6<c64, 32>", "spread_kernel_w6<HIP_vector_type<f
                ("interp_pair_kernel<6, 16, 16, 16>", "interp_pair_kernel<6, 16, 16, 16>", 40_000_000))
 
 
-def summarize_sq(dir_a, dir_b, workload):
+# the Toeplitz passes of the A^H A apply (full-size launches: the grid of the
+# fused passes is n_z x coil groups x threads, of P3 at least 2^20 work-items)
+_SQ_TOEP_PASSES = ("SQ_INSTS_VALU SQ_INSTS_LDS SQ_WAIT_INST_ANY SQ_WAVE_CYCLES SQ_BUSY_CYCLES SQ_ACTIVE_INST_VALU "
+                   "SQ_ACTIVE_INST_LDS SQ_LDS_BANK_CONFLICT",
+                   "SQ_WAVES SQ_INSTS_VMEM_RD SQ_INSTS_VMEM_WR SQ_WAIT_ANY SQ_INSTS_SALU GRBM_GUI_ACTIVE")
+_SQ_TOEP_KERNELS = (("toep_F12_xy_pad_fft", "toep_pad_fft_kernel", 100_000),
+                    ("toep_P3_z_psf", "fft2_pass_kernel<HIP_vector_type<float, 2u>, 16, 16, -1, 1, 3>", 1_000_000),
+                    ("toep_F45_yx_ifft_crop_sum", "toep_crop_sum_kernel", 100_000))
+
+
+def summarize_sq(dir_a, dir_b, workload, toeplitz=False):
     """Markdown summary of the two SQ-counter passes over the NUFFT spread and
-    interpolation kernels (full-size launches only)."""
+    interpolation kernels, or (toeplitz) over the passes of the A^H A apply
+    (full-size launches only)."""
+    kernels = _SQ_TOEP_KERNELS if toeplitz else _SQ_KERNELS
+    passes = _SQ_TOEP_PASSES if toeplitz else _SQ_PASSES
     import collections
     import csv
     import glob
@@ -271,19 +284,22 @@ def summarize_sq(dir_a, dir_b, workload):
         agg = collections.defaultdict(lambda: collections.defaultdict(list))
         for f in glob.glob(os.path.join(d, "**", "*counter_collection*.csv"), recursive=True):
             for r in csv.DictReader(open(f)):
-                for label, sub, min_grid in _SQ_KERNELS:
+                for label, sub, min_grid in kernels:
                     if sub in r["Kernel_Name"] and int(r["Grid_Size"]) >= min_grid:
                         agg[label][r["Counter_Name"]].append(float(r["Counter_Value"]))
         for label, counters in agg.items():
             for c, vals in counters.items():
                 res[label][c] = sum(vals) / len(vals)
     cmd = ("cd /tmp && export TMPDIR=/tmp && cd $REPO && rocprofv3 --pmc {c} --output-format csv -d {d} -o run "
-           "-- python -u bench.py --steps 1 --warmup 0 --no-cpu-baseline --phases type2,type1")
-    labels = [k for k, _, _ in _SQ_KERNELS if k in res]
-    out = [f"# SQ counters: NUFFT spread and interpolation, {workload}", "",
+           "-- python -u bench.py --steps 1 --warmup 0 --no-cpu-baseline --phases "
+           + ("aha" if toeplitz else "type2,type1"))
+    labels = [k for k, _, _ in kernels if k in res]
+    title = "Toeplitz passes of the A^H A apply" if toeplitz else "NUFFT spread and interpolation"
+    flag = "--summarize-sq-toeplitz" if toeplitz else "--summarize-sq"
+    out = [f"# SQ counters: {title}, {workload}", "",
            "Two counter passes (the hardware takes at most 8 SQ counters per pass), from the repository root on one "
-           "MI355X, then the summary:", "", "```", cmd.format(c=_SQ_PASSES[0], d=dir_a), cmd.format(c=_SQ_PASSES[1], d=dir_b),
-           f"python bench.py --summarize-sq {dir_a} {dir_b}", "```", "",
+           "MI355X, then the summary:", "", "```", cmd.format(c=passes[0], d=dir_a), cmd.format(c=passes[1], d=dir_b),
+           f"python bench.py {flag} {dir_a} {dir_b}", "```", "",
            "Values are per launch (mean over the full-size launches), in units of 10^9. Instruction counts are "
            "wave-instructions summed over the GPU; `GRBM_GUI_ACTIVE` is summed over the 8 XCDs.", "",
            "| counter | " + " | ".join(f"`{k}`" for k in labels) + " |", "|---|" + "---:|" * len(labels)]
@@ -295,7 +311,11 @@ def summarize_sq(dir_a, dir_b, workload):
         if v and f:
             out.append(f"- `{k}`: VALU-issue floor {v / 256 / 2.4e9 * 1e3:.1f} ms (one wave-instruction per CU per "
                        f"cycle, 256 CUs, 2.4 GHz); FP32 FMAs are {100 * f / v:.0f}% of its VALU instructions.")
-    path = os.path.join(ROOT, "profiles", f"nufft_spread_interp_sq_counters_{workload}.md")
+        elif v:
+            out.append(f"- `{k}`: VALU-issue floor {v / 256 / 2.4e9 * 1e3:.2f} ms (one wave-instruction per CU per "
+                       f"cycle, 256 CUs, 2.4 GHz).")
+    name = "toeplitz_passes_sq_counters" if toeplitz else "nufft_spread_interp_sq_counters"
+    path = os.path.join(ROOT, "profiles", f"{name}_{workload}.md")
     with open(path, "w") as fh:
         fh.write("\n".join(out) + "\n")
     print(path)
@@ -392,6 +412,9 @@ def main():
     ap.add_argument("--summarize-sq", nargs=2, metavar=("PASS1_DIR", "PASS2_DIR"),
                     help="summarise the two SQ-counter passes over the NUFFT kernels into "
                          "profiles/nufft_spread_interp_sq_counters_<workload>.md and exit")
+    ap.add_argument("--summarize-sq-toeplitz", nargs=2, metavar=("PASS1_DIR", "PASS2_DIR"),
+                    help="summarise the two SQ-counter passes over the Toeplitz passes into "
+                         "profiles/toeplitz_passes_sq_counters_<workload>.md and exit")
     args = ap.parse_args()
     if args.reduce_pmc:
         reduce_pmc(args.reduce_pmc[0], args.reduce_pmc[1], args.workload)
@@ -401,6 +424,9 @@ def main():
         return
     if args.summarize_sq:
         summarize_sq(args.summarize_sq[0], args.summarize_sq[1], args.workload)
+        return
+    if args.summarize_sq_toeplitz:
+        summarize_sq(args.summarize_sq_toeplitz[0], args.summarize_sq_toeplitz[1], args.workload, toeplitz=True)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
