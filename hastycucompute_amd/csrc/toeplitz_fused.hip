@@ -313,6 +313,17 @@ toep_pad_fft_kernel(ToepPadFftArgs a, const float2* tw, int groups) {
         }
     };
     if (b0 < b1) load_in(b0);
+    // a broadcast first diagonal (the image of a SENSE apply) is read once
+    // per block, not once per coil
+    const bool m1_bcast = a.m1.ptr && a.m1.sb == 0;
+    V d1[H][16];
+    if (m1_bcast) {
+#pragma unroll
+        for (int h = 0; h < H; ++h)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) d1[h][i] = make_float2(1.f, 0.f);
+        pad_fft_diag<H>(d1, a.m1, 0, z, yb0, s0);
+    }
     __syncthreads();
 
 #pragma unroll 1
@@ -320,7 +331,14 @@ toep_pad_fft_kernel(ToepPadFftArgs a, const float2* tw, int groups) {
         {
             int yb = yb0, s = s0;
             asm volatile("" : "+v"(yb), "+v"(s));
-            pad_fft_diag<H>(xs, a.m1, b, z, yb, s);
+            if (m1_bcast) {
+#pragma unroll
+                for (int h = 0; h < H; ++h)
+#pragma unroll
+                    for (int i = 0; i < 16; ++i) xs[h][i] = cmul(xs[h][i], d1[h][i]);
+            } else {
+                pad_fft_diag<H>(xs, a.m1, b, z, yb, s);
+            }
             pad_fft_diag<H>(xs, a.m2, b, z, yb, s);
         }
         char* gp = reinterpret_cast<char*>(a.grid + int64_t(b) * a.grid_b + int64_t(z) * (kLY * kLX));
