@@ -44,7 +44,7 @@ static ToeplitzDims toeplitz_dims(int dim, const int64_t* n) {
 }
 
 // The fused passes (toeplitz_fused.hip) run coil-summed 3-D applies of at
-// least two coils on 128 x 128 planes: P1 + P2 in one kernel writing x
+// least two coils on n x n planes (n = 64, 96, 128): P1 + P2 in one kernel writing x
 // class-major, P3 reading the kernel through the same permutation, P4 + P5
 // in one kernel with the coil sum.  HASTY_TOEP_FUSED=0 turns them off, =2
 // keeps only the last one (P1 then stores class-major itself);
@@ -81,7 +81,7 @@ static void toeplitz_core(const ToeplitzDims& d, int64_t B, const float2* input,
         f.in = input; f.in_b = d.nimg();
         f.m1 = in1; f.m2 = in2;
         f.grid = bufB; f.grid_b = d.nB();
-        f.nbatch = int(B); f.nz = int(nz);
+        f.nbatch = int(B); f.nz = int(nz); f.n = int(nx);
         launch_toep_pad_fft(f, s);
     }
     // P1
@@ -129,6 +129,7 @@ static void toeplitz_core(const ToeplitzDims& d, int64_t B, const float2* input,
             f.nbatch = int(B); f.nz = int(nz);
             f.groups = toep_fused_groups(nz, B);
             f.accumulate = accumulate;
+            f.n = int(nx);
             launch_toep_crop_sum(f, s);
             return;
         }

@@ -1,6 +1,7 @@
-// Fused last pass of the Toeplitz-embedded SENSE normal operator (3-D,
-// 128 x 128 planes): y inverse FFT + crop, x inverse FFT + crop, scale,
-// output diagonals and the coil sum in one kernel.  Host interface.
+// Fused passes of the Toeplitz-embedded SENSE normal operator (3-D, n x n
+// planes with n = 64, 96 or 128): the first two (x and y zero-padded FFTs)
+// and the last two (y and x inverse FFTs with crops, scale, output diagonals
+// and the coil sum), each as one kernel.  Host interface.
 #pragma once
 
 #include "fft.hpp"
@@ -16,6 +17,7 @@ struct ToepCropSumArgs {
     DiagOp<float2> m1, m2, io;   // element (b, row, x) at ptr[b*sb + row*sr + x], row = z*ny + y
     float scale;
     int nbatch, nz, groups, accumulate;
+    int n;                   // plane size ny == nx
 };
 
 // First two passes fused: x and y zero-padded forward FFTs of
@@ -27,10 +29,11 @@ struct ToepPadFftArgs {
     float2* grid;            // [B, nz, 2ny, 2nx], x positions class-major
     int64_t grid_b;
     int nbatch, nz;
+    int n;                   // plane size ny == nx
 };
 void launch_toep_pad_fft(const ToepPadFftArgs& a, hipStream_t s);
 
-// plane shapes the fused passes are built for (ny == nx == 128)
+// plane shapes the fused passes are built for (ny == nx, 64, 96 or 128)
 bool toep_crop_sum_supported(int64_t ny, int64_t nx);
 // groups: coil groups per plane (partial sums added by a second kernel)
 void launch_toep_crop_sum(const ToepCropSumArgs& a, hipStream_t s);

@@ -119,39 +119,43 @@ def test_cartesian_full_sampling_is_scaled_identity():
     report("Cartesian T = I", rel_l2(host(g), x), TOL)
 
 
-# (nz, coils, coil_batch, HASTY_TOEP_GROUPS): 128 x 128 planes take the fused
-# last pass; the cases cover one coil per block (partial sums), several coils
-# per block with and without partial sums, an empty trailing group, and the
-# chunked accumulation of a second coil batch
-FUSED_CASES = [(4, 3, 0, None), (4, 3, 2, None), (4, 5, 0, "2"), (4, 4, 0, "1"), (3, 7, 0, "3"),
-               (2, 2, 0, "5")]
+# (n, nz, coils, coil_batch, HASTY_TOEP_GROUPS): n x n planes (n = 64, 96,
+# 128) take the fused passes; the cases cover one coil per block (partial
+# sums), several coils per block with and without partial sums, an empty
+# trailing group, and the chunked accumulation of a second coil batch
+FUSED_CASES = [(128, 4, 3, 0, None), (128, 4, 3, 2, None), (128, 4, 5, 0, "2"), (128, 4, 4, 0, "1"),
+               (128, 3, 7, 0, "3"), (128, 2, 2, 0, "5"), (96, 3, 3, 0, None), (96, 4, 5, 2, "2"),
+               (96, 2, 4, 0, "1"), (64, 4, 3, 0, None), (64, 3, 5, 0, "2")]
 
 
-@pytest.mark.parametrize("nz,C,chunk,groups", FUSED_CASES)
-def test_normal_op_fused_last_pass(nz, C, chunk, groups, monkeypatch):
-    """y and x inverse FFTs, crops, maps and coil sum in one kernel (x stored
-    class-major by the first pass, the kernel read through the same
-    permutation) vs the FP64 oracle, with a random kernel."""
+@pytest.mark.parametrize("n,nz,C,chunk,groups", FUSED_CASES)
+def test_normal_op_fused_last_pass(n, nz, C, chunk, groups, monkeypatch):
+    """The fused passes (x and y pad FFTs in one kernel writing x
+    class-major; y and x inverse FFTs, crops, maps and coil sum in another; the
+    kernel read through the same permutation) vs the FP64 oracle, with a
+    random kernel."""
     from hastycucompute_amd.mri import NORMAL_T_T1_OP
     if groups is not None:
         monkeypatch.setenv("HASTY_TOEP_GROUPS", groups)
-    shape = (nz, 128, 128)
+    shape = (nz, n, n)
     rng = np.random.default_rng(11 + nz + C)
     S = crandn(rng, (C,) + shape)
     x = crandn(rng, shape)
     K = crandn(rng, tuple(2 * n for n in shape))
     op = NORMAL_T_T1_OP(cuda(K), cuda(S), coil_batch=chunk)
     y = host(op(cuda(x)))
-    report(f"fused last pass nz={nz} C={C} chunk={chunk} groups={groups}",
+    report(f"fused passes n={n} nz={nz} C={C} chunk={chunk} groups={groups}",
            rel_l2(y, sense_normal(x, S, K)), TOL)
 
 
-def test_normal_idt_fused_last_pass():
-    """Two kernels with spatial diagonals R_l (conj(R_l) applied in the fused
-    pass, the second kernel accumulated) on 128 x 128 planes."""
+@pytest.mark.parametrize("n", [128, 96])
+def test_normal_idt_fused_last_pass(n):
+    """Two kernels with spatial diagonals R_l (R_l applied in the fused first
+    pass, conj(R_l) in the fused last pass, the second kernel accumulated) on
+    n x n planes."""
     from hastycucompute_amd import mri
     from oracle import normal_ops as no
-    shape = (3, 128, 128)
+    shape = (3, n, n)
     rng = np.random.default_rng(12)
     C, L = 3, 2
     S = crandn(rng, (C,) + shape)
@@ -159,4 +163,4 @@ def test_normal_idt_fused_last_pass():
     K = crandn(rng, (L,) + tuple(2 * n for n in shape))
     R = np.exp(1j * rng.uniform(-np.pi, np.pi, (L,) + shape)).astype(np.complex64)
     op = mri.NORMAL_IDT_T1_OP(cuda(K), cuda(R), cuda(S))
-    report("fused last pass NORMAL_IDT_T1_OP L=2", rel_l2(host(op(cuda(x))), no.normal_idt_t1(x, S, K, R)), TOL)
+    report(f"fused passes NORMAL_IDT_T1_OP n={n} L=2", rel_l2(host(op(cuda(x))), no.normal_idt_t1(x, S, K, R)), TOL)
