@@ -137,7 +137,7 @@ def _read(r, device):
     if t == TENSOR:
         dev_type, code, ndim, dev_idx, total = _HEADER.unpack(r.take(_HEADER.size, "tensor header"))
         shape = list(struct.unpack(f"<{ndim}q", r.take(8 * ndim, "shape")))
-        if int(np.prod(shape, dtype=np.int64)) != total:
+        if any(d < 0 for d in shape) or total < 0 or int(np.prod(shape, dtype=object)) != total:
             raise RuntimeError("Tensor shape does not match total elements in header")
         if code not in _CODE_SCALAR:
             raise RuntimeError(f"unknown scalar type code {code}")

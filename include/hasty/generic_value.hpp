@@ -301,9 +301,17 @@ private:
                 int64_t total;
                 std::memcpy(&total, h + 8, 8);
                 t.shape.resize(h[2]);
-                for (auto& s : t.shape) s = r.i64("shape");
-                if (t.numel() != total) throw std::runtime_error("Tensor shape does not match total elements in header");
-                const size_t nb = size_t(total) * detail_gv::element_size(t.scalar_type);
+                int64_t count = 1;   // checked product: a corrupt header must not overflow it
+                for (auto& s : t.shape) {
+                    s = r.i64("shape");
+                    if (s < 0 || (s != 0 && count > INT64_MAX / s))
+                        throw std::runtime_error("Tensor shape does not match total elements in header");
+                    count *= s;
+                }
+                if (count != total) throw std::runtime_error("Tensor shape does not match total elements in header");
+                const size_t es = detail_gv::element_size(t.scalar_type);
+                if (size_t(total) > (r.n - r.pos) / es) throw std::runtime_error("Buffer too small to deserialize tensor data");
+                const size_t nb = size_t(total) * es;
                 const uint8_t* d = r.take(nb, "tensor data");
                 t.bytes.assign(d, d + nb);
                 return generic_value(std::move(t));

@@ -154,6 +154,7 @@ public:
     }
 
     size_t nthreads() const { return _storages.size(); }
+    // the workers' storages; read them while no task runs (tasks change them)
     const std::vector<storage>& storages() const { return _storages; }
 
     // Waits for every task enqueued before the call, then removes

@@ -105,6 +105,10 @@ def test_errors():
     bad[16:24] = struct.pack("<q", 4)          # total_elements 4 for shape (3,)
     with pytest.raises(RuntimeError, match="Tensor shape does not match total elements in header"):
         gv.deserialize(bytes(bad))
+    bad = bytearray(b)
+    bad[24:32] = struct.pack("<q", -3)         # a negative size
+    with pytest.raises(RuntimeError, match="Tensor shape does not match total elements in header"):
+        gv.deserialize(bytes(bad))
     with pytest.raises(RuntimeError, match="unknown generic_value type tag 9"):
         gv.deserialize(struct.pack("<q", 9))
     with pytest.raises(TypeError):
