@@ -185,7 +185,7 @@ _TRACE_CMD = ("cd /tmp && export TMPDIR=/tmp && cd $REPO && rocprofv3 --kernel-t
 _TIMED_SEQ = ([("toep_F12_xy_pad_fft", "toep_pad_fft_kernel"),
                ("toep_P3_z_psf", "<c64, 16, 16, -1, 1, 3>"), ("toep_F45_yx_ifft_crop_sum", "toep_crop_sum_kernel"),
                ("toep_F45 sum_partials", "sum_partials")] +
-              [("fwd_x_pad_fft", "<c64, 16, 16, -1, 2, 1>"), ("fwd_y_pad_fft", "<c64, 16, 16, -1, 0, 1>"),
+              [("fwd_xy_pad_fft", "nufft_fwd_xy_kernel"),
                ("fwd_z_pad_fft", "<c64, 16, 16, -1, 0, 1>"), ("nufft_interp", "interp_pair_kernel<6,")] +
               [("nufft_sort_values", "sort_values_kernel<c64"), ("nufft_spread", "spread_kernel_w6<c64, 32>"),
                ("nufft_spread_reduce", "reduce_split_tiles"), ("inv_z_fft_crop", "<c64, 16, 16, 1, 0, 4>"),

@@ -19,13 +19,13 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhasty_hip.so")
 SOURCES = ["fft.hip", "fft_reg_f32.hip", "fft_reg_f64.hip", "nufft.hip", "nufft_kern_f32.hip",
-           "nufft_kern_f64.hip", "toeplitz.hip", "toeplitz_fused.hip", "cg.hip", "abi.cpp"]
+           "nufft_kern_f64.hip", "toeplitz.hip", "toeplitz_fused.hip", "nufft_fused.hip", "cg.hip", "abi.cpp"]
 ARCH = os.environ.get("HASTY_OFFLOAD_ARCH", "gfx950")
 # Per-file flags.  The register FFT passes fold arithmetic on known-zero
 # (padded) inputs away, which needs x + 0 -> x: signed zeros do not matter
 # to an FFT, so those files are built without them.
 EXTRA_FLAGS = {"fft_reg_f32.hip": ["-fno-signed-zeros"], "fft_reg_f64.hip": ["-fno-signed-zeros"],
-               "toeplitz_fused.hip": ["-fno-signed-zeros"]}
+               "toeplitz_fused.hip": ["-fno-signed-zeros"], "nufft_fused.hip": ["-fno-signed-zeros"]}
 
 
 def hipcc() -> str:

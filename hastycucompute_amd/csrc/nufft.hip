@@ -10,6 +10,7 @@
 
 #include "kb.hpp"
 #include "nufft.hpp"
+#include "toeplitz_fused.hpp"
 #include "nufft_kernels.hpp"
 
 #include <hipcub/hipcub.hpp>
@@ -39,12 +40,39 @@ template <class V> int rows_per_block(int L, int out_len) {
     return int(std::max<int64_t>(1, nl));
 }
 
+// The x and y passes of 3-D c64 plans with 128 x 128 planes on a 256 x 256
+// fine grid run as one fused kernel (nufft_fused.hip); HASTY_NUFFT_FUSED=0
+// keeps the separate passes.
+static bool nufft_fused_enabled() {
+    static const bool v = [] {
+        const char* e = std::getenv("HASTY_NUFFT_FUSED");
+        return !(e && e[0] == '0');
+    }();
+    return v;
+}
+
 template <class V>
 void pruned_forward(const PadGeometry& g, int64_t B, const V* img, V* grid, V* bufA, V* bufB,
                     int sign, const DiagOp<V>& m1, const DiagOp<V>& m2,
                     const decltype(V::x)* const* deapod, hipStream_t s) {
     const int64_t nz = g.n[0], ny = g.n[1], nx = g.n[2];
     const int64_t Lz = g.L[0], Ly = g.L[1], Lx = g.L[2];
+    bool fused_xy = false;
+    if constexpr (std::is_same_v<V, float2>) {
+        fused_xy = Lz > 1 && B >= 1 && nufft_fused_enabled() && !fft_force_stockham() &&
+                   nufft_fwd_xy_supported(ny, nx, Ly, Lx, g.shift_in[1], g.shift_in[2]) &&
+                   (!deapod || (deapod[0] && deapod[1] && deapod[2]));
+        if (fused_xy) {
+            NufftFwdXyArgs f{};
+            f.in = img; f.in_b = nz * ny * nx;
+            f.m1 = m1; f.m2 = m2;
+            f.dz = deapod ? deapod[0] : nullptr; f.dy = deapod ? deapod[1] : nullptr;
+            f.dx = deapod ? deapod[2] : nullptr;
+            f.grid = bufB; f.grid_b = nz * Ly * Lx;
+            f.nbatch = int(B); f.nz = int(nz); f.sign = sign;
+            launch_nufft_fwd_xy(f, s);
+        }
+    }
     // x: image rows -> bufA [B, nz, ny, Lx]
     RowLoadArgs<V> r{};
     r.in = img; r.out = bufA;
@@ -59,7 +87,7 @@ void pruned_forward(const PadGeometry& g, int64_t B, const V* img, V* grid, V* b
     r.plan = make_fft_plan(int(Lx));
     r.tw = reinterpret_cast<const V*>(twiddles(int(Lx), sizeof(V) == 16));
     r.tag = "fwd_x_pad_fft";
-    launch_row_load<V>(r, s);
+    if (!fused_xy) launch_row_load<V>(r, s);
     // y: bufA -> (3-D) bufB [B, nz, Ly, Lx] or (2-D) grid [B, Ly, Lx]
     ColArgs<V> c = make_col_args<V>(int(Ly));
     c.in = bufA; c.ni = int(Lx); c.no = int(nz); c.nbatch = int(B);
@@ -69,7 +97,7 @@ void pruned_forward(const PadGeometry& g, int64_t B, const V* img, V* grid, V* b
     c.out_s = Lx; c.out_o = Ly * Lx; c.out_b = nz * Ly * Lx;
     c.out_len = int(Ly); c.out_shift = 0; c.sign = sign;
     c.tag = "fwd_y_pad_fft";
-    launch_col_pass<V>(c, s);
+    if (!fused_xy) launch_col_pass<V>(c, s);
     if (Lz == 1) return;
     // z: bufB -> grid [B, Lz, Ly, Lx]
     ColArgs<V> z = make_col_args<V>(int(Lz));

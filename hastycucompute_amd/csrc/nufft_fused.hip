@@ -1,0 +1,190 @@
+// Fused x and y passes of the NUFFT type-2 (forward) fine-grid transform on
+// gfx950 for 3-D c64 plans with n_y = n_x = 128 on a 256 x 256 fine grid
+// (upsampling 2): the pruned x pass and y pass in one kernel, so the
+// [B, n_z, n_y, 256] intermediate between them never goes through HBM.
+//
+// The image holds centred modes (index j <-> mode j - 64, fine-grid position
+// j - 64 mod 256) times the deapodisation dz dy dx and the load diagonals.
+// The y transform splits by residue classes of ky = 4m + r, so that x stays
+// in natural order for the interpolation that reads the grid:
+//     X[4m + r] = sum_{n'<64} e^{s 2 pi i r n'/256} (g[n' + 64] + c_r g[n'])
+//                 e^{s 2 pi i m n'/64},   c_r = e^{-s pi i r / 2},
+// (s the plan's sign; g[j] the column, j = n' + 64 the modes n' >= 0 and
+// j = n' the modes n' - 64 < 0).  Per plane z, coil b and class r:
+//   Y  each column x: fold, 64-point DFT (8 x 8, one LDS exchange) -> rows m;
+//   X  each row m: half-input 256-point DFT along x (16 x 16, one
+//      exchange), times e^{-s pi i kx / 2} for the -64 shift, staged in LDS
+//      and stored as whole 2-KB fine-grid rows ky = 4m + r.
+#include <algorithm>
+
+#include "fft_reg.hpp"
+#include "toeplitz_fused.hpp"
+
+namespace hasty {
+
+namespace {
+constexpr int kN = 128, kL = 256, kM = 64;
+constexpr int kT = 1024;
+constexpr int kR2 = 72 * kN;           // y exchange [s 9 + k1][x]
+constexpr int kRS1 = kN + 1;           // class rows [m][x], odd stride
+constexpr int kR1 = kM * kRS1;
+constexpr int kOS = kL + 1;            // staged output rows [m][kx]
+constexpr int kE = 16 * 16 * kM;       // x exchange [k1][p2][m]
+constexpr int kLdsElems = std::max(kR2 + kR1, std::max(kE, kM * kOS)) + kL;
+}  // namespace
+
+template <int SIGN>
+__global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(1, 4)))
+nufft_fwd_xy_kernel(NufftFwdXyArgs a, const float2* tw) {
+    using V = float2;
+    extern __shared__ unsigned char smem_raw[];
+    V* lds = reinterpret_cast<V*>(smem_raw);
+    V* R2 = lds;               // y exchange
+    V* R1 = lds + kR2;         // class rows
+    V* E = lds;                // x exchange (R1 / R2 are read by then)
+    V* O = lds;                // staged output rows (E is read by then)
+    V* twl = lds + (kLdsElems - kL);
+    const int tid = threadIdx.x;
+    for (int k = tid; k < kL; k += kT) twl[k] = tw[k];
+
+    const int z = int(blockIdx.x % unsigned(a.nz));
+    const int g = int(blockIdx.x / unsigned(a.nz));
+    const int cpb = (a.nbatch + a.groups - 1) / a.groups;
+    const int b0 = g * cpb;
+    const int b1 = min(a.nbatch, b0 + cpb);
+    // phase Y roles: column x, lane s of the 8 x 8 DFT (s wave-uniform)
+    const int x0 = tid & (kN - 1), s0 = tid >> 7;
+    // phase X roles: row m, line p2 / k1 of the 16 x 16 DFT (wave-uniform)
+    const int mm0 = tid & (kM - 1), pa0 = tid >> 6;
+
+    // g[h][p1]: column x0, rows j = 8 p1 + s (h = 0) and 8 p1 + s + 64 (h = 1)
+    V gx[2][8];
+    auto load_in = [&](int b) {
+        int x = x0, s = s0;
+        asm volatile("" : "+v"(x), "+v"(s));
+        const char* pl = reinterpret_cast<const char*>(a.in + int64_t(b) * a.in_b + int64_t(z) * (kN * kN));
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int p1 = 0; p1 < 8; ++p1)
+                gx[h][p1] = load_at<V>(pl, uint32_t(((8 * p1 + s + 64 * h) * kN + x) * int(sizeof(V))));
+    };
+    auto apply = [&](const DiagOp<V>& d, int b) {
+        if (!d.ptr) return;
+        int x = x0, s = s0;
+        asm volatile("" : "+v"(x), "+v"(s));
+        const float sgn = d.type == HASTY_MULT_CONJ ? -1.f : 1.f;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            V w[8];
+#pragma unroll
+            for (int p1 = 0; p1 < 8; ++p1)
+                w[p1] = d.ptr[int64_t(b) * d.sb + (int64_t(z) * kN + 8 * p1 + s + 64 * h) * d.sr + x];
+#pragma unroll
+            for (int p1 = 0; p1 < 8; ++p1) gx[h][p1] = cmul_sgn(gx[h][p1], w[p1], sgn);
+        }
+    };
+    if (b0 < b1) load_in(b0);
+    __syncthreads();
+
+#pragma unroll 1
+    for (int b = b0; b < b1; ++b) {
+        apply(a.m1, b);
+        apply(a.m2, b);
+        if (a.dx) {   // deapodisation dz[z] dy[j] dx[x]
+            int x = x0, s = s0;
+            asm volatile("" : "+v"(x), "+v"(s));
+            const float zx = a.dz[z] * a.dx[x];
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int p1 = 0; p1 < 8; ++p1) gx[h][p1] = cscale(gx[h][p1], zx * a.dy[8 * p1 + s + 64 * h]);
+        }
+        char* gp = reinterpret_cast<char*>(a.grid + int64_t(b) * a.grid_b + int64_t(z) * (kL * kL));
+#pragma unroll 1
+        for (int r = 0; r < 4; ++r) {
+            int x = x0, s = s0, mm = mm0, pa = pa0;
+            asm volatile("" : "+v"(x), "+v"(s), "+v"(mm), "+v"(pa));
+            // ---- Y: fold the centred column by class r, 64-point DFT ----
+            V u[8];
+#pragma unroll
+            for (int p1 = 0; p1 < 8; ++p1) {
+                const int np = 8 * p1 + s;
+                // c_r = e^{-s pi i r / 2} = i^{-s r}
+                V t = cadd(gx[1][p1], quarter_turn(gx[0][p1], SIGN < 0 ? r : 4 - r));
+                u[p1] = r ? twiddle_mul<V, SIGN>(t, twl, (r * np) & (kL - 1)) : t;
+            }
+            if (r == 3 && b + 1 < b1) load_in(b + 1);   // this coil's inputs are folded
+            RegDFT<8, SIGN, V>::run(u);
+#pragma unroll
+            for (int k1 = 1; k1 < 8; ++k1) u[k1] = twiddle_mul<V, SIGN>(u[k1], twl, (4 * s * k1) & (kL - 1));
+            __syncthreads();   // the previous class's staged rows are stored
+#pragma unroll
+            for (int k1 = 0; k1 < 8; ++k1) R2[(s * 9 + k1) * kN + x] = u[k1];
+            __syncthreads();
+#pragma unroll
+            for (int p2 = 0; p2 < 8; ++p2) u[p2] = R2[(p2 * 9 + s) * kN + x];
+            RegDFT<8, SIGN, V>::run(u);   // u[k2] = X_r[m = s + 8 k2][x]
+#pragma unroll
+            for (int k2 = 0; k2 < 8; ++k2) R1[(s + 8 * k2) * kRS1 + x] = u[k2];
+            __syncthreads();
+            // ---- X: half-input 256-point DFT of each class row ----
+            V v[16];
+#pragma unroll
+            for (int p1 = 0; p1 < 16; ++p1) v[p1] = p1 < 8 ? R1[mm * kRS1 + 16 * p1 + pa] : make_float2(0.f, 0.f);
+            RegDFT<16, SIGN, V>::run(v);
+#pragma unroll
+            for (int k1 = 1; k1 < 16; ++k1) v[k1] = twiddle_mul<V, SIGN>(v[k1], twl, (pa * k1) & (kL - 1));
+            __syncthreads();   // R1 is read; E overwrites it
+#pragma unroll
+            for (int k1 = 0; k1 < 16; ++k1) E[(k1 * 16 + pa) * kM + mm] = v[k1];
+            __syncthreads();
+#pragma unroll
+            for (int p2 = 0; p2 < 16; ++p2) v[p2] = E[(pa * 16 + p2) * kM + mm];
+            RegDFT<16, SIGN, V>::run(v);   // v[k2] = kx = pa + 16 k2, before the shift
+            // the -64 shift: output kx times e^{-s pi i kx / 2} = i^{-s (kx mod 4)}, kx mod 4 = pa mod 4
+            const int q = SIGN < 0 ? (pa & 3) : ((4 - (pa & 3)) & 3);
+#pragma unroll
+            for (int k2 = 0; k2 < 16; ++k2) v[k2] = quarter_turn(v[k2], q);
+            __syncthreads();   // E is read; O overwrites it
+#pragma unroll
+            for (int k2 = 0; k2 < 16; ++k2) O[mm * kOS + pa + 16 * k2] = v[k2];
+            __syncthreads();
+            // ---- store: whole fine-grid rows ky = 4m + r ----
+            {
+                const int kx = tid & (kL - 1), m4 = tid >> 8;   // 4 rows per round
+#pragma unroll 1
+                for (int m = m4; m < kM; m += kT / kL)
+                    store_at<V, true>(gp, uint32_t(((4 * m + r) * kL + kx) * int(sizeof(V))), O[m * kOS + kx]);
+            }
+        }
+    }
+}
+
+bool nufft_fwd_xy_supported(int64_t ny, int64_t nx, int64_t Ly, int64_t Lx, int shift_y, int shift_x) {
+    return ny == kN && nx == kN && Ly == kL && Lx == kL && shift_y == -kN / 2 && shift_x == -kN / 2;
+}
+
+void launch_nufft_fwd_xy(const NufftFwdXyArgs& a, hipStream_t s) {
+    HASTY_REQUIRE(a.nz >= 1 && a.nbatch >= 1, "fused fine-grid pass: empty batch or plane count");
+    HASTY_REQUIRE(a.sign == 1 || a.sign == -1, "fused fine-grid pass: sign must be +1 or -1");
+    HASTY_REQUIRE(a.grid_b >= int64_t(a.nz) * kL * kL, "fused fine-grid pass: grid stride too small");
+    const size_t bytes = size_t(kLdsElems) * sizeof(float2);
+    auto kern = a.sign < 0 ? nufft_fwd_xy_kernel<-1> : nufft_fwd_xy_kernel<1>;
+    static const bool lds_ok = [&] {
+        for (auto k : {nufft_fwd_xy_kernel<-1>, nufft_fwd_xy_kernel<1>})
+            HASTY_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k),
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)));
+        return true;
+    }();
+    (void)lds_ok;
+    const int groups = int(std::max<int64_t>(1, std::min<int64_t>((256 + a.nz - 1) / a.nz, a.nbatch)));
+    NufftFwdXyArgs b = a;
+    b.groups = groups;
+    const float2* tw = reinterpret_cast<const float2*>(twiddles(kL, false));
+    HASTY_PROF("fwd_xy_pad_fft", s);
+    hipLaunchKernelGGL(kern, dim3(unsigned(int64_t(a.nz) * groups)), dim3(kT), bytes, s, b, tw);
+    HASTY_CHECK_HIP(hipGetLastError());
+}
+
+}  // namespace hasty

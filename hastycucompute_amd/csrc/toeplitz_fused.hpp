@@ -33,6 +33,21 @@ struct ToepPadFftArgs {
 };
 void launch_toep_pad_fft(const ToepPadFftArgs& a, hipStream_t s);
 
+// NUFFT type-2 fine grid: the x and y zero-padded forward passes of a 3-D c64
+// plan with n_y = n_x = 128 on 256 x 256 (centred modes, shift -64), fused
+// (nufft_fused.hip); img [B, nz, 128, 128] -> grid [B, nz, 256, 256]
+struct NufftFwdXyArgs {
+    const float2* in;
+    int64_t in_b;
+    DiagOp<float2> m1, m2;                         // load diagonals, as the row pass takes them
+    const float* dz; const float* dy; const float* dx;   // deapodisation (all or none)
+    float2* grid;
+    int64_t grid_b;
+    int nbatch, nz, sign, groups;
+};
+bool nufft_fwd_xy_supported(int64_t ny, int64_t nx, int64_t Ly, int64_t Lx, int shift_y, int shift_x);
+void launch_nufft_fwd_xy(const NufftFwdXyArgs& a, hipStream_t s);
+
 // plane shapes the fused passes are built for (ny == nx, 64, 96 or 128)
 bool toep_crop_sum_supported(int64_t ny, int64_t nx);
 // groups: coil groups per plane (partial sums added by a second kernel)

@@ -277,3 +277,38 @@ def test_setpts_is_deterministic():
     assert all(torch.equal(outs1[0], o) for o in outs1[1:])
     assert all(torch.equal(outs2[0], o) for o in outs2[1:])
     print("[parity] setpts: bitwise identical type 1 / type 2 over repeated setpts and plans", flush=True)
+
+
+@pytest.mark.parametrize("sign,ntransf", [(-1, 2), (1, 3)])
+def test_fused_fine_grid_xy_pass_128_planes(sign, ntransf):
+    """Type 2 with 128 x 128 planes on a 256 x 256 fine grid runs its x and y
+    passes as one kernel (y split by residue classes, x in natural order, the
+    centred-mode shift folded in): vs FP64 NDFT, both signs, batched; the
+    type-1 round trip of the same plan pair is unchanged."""
+    shape = (4, 128, 128)
+    rng = np.random.default_rng(21 + ntransf)
+    M = 1500
+    coords = rand_coords(rng, 3, M)
+    f = crandn(rng, (ntransf,) + shape)
+    c = crandn(rng, (ntransf, M))
+    out1, out2 = _run(shape, coords, f, c, ntransf, sign=sign)
+    report(f"fused fine-grid type 2 {shape} sign={sign} T={ntransf}",
+           rel_l2(out2, ndft_type2(f, coords, sign=sign)), TOL)
+    report(f"type 1 {shape} sign={sign}", rel_l2(out1, ndft_type1(c, coords, shape, sign=sign)), TOL)
+
+
+def test_fused_fine_grid_sense_forward_128_planes():
+    """SENSE forward on 128 x 128 planes: maps and image ride in the fused
+    pass's load diagonals (as in the row pass) vs the FP64 NDFT model."""
+    from hastycucompute_amd import mri
+    from oracle.solvers import sense_forward
+    shape = (3, 128, 128)
+    rng = np.random.default_rng(5)
+    coords = rand_coords(rng, 3, 1200)
+    S = crandn(rng, (3,) + shape)
+    x = crandn(rng, shape)
+    bil = np.ones((1, 1200), np.complex64)
+    cjl = np.ones((1,) + shape, np.complex64)
+    op = mri.SenseOffresonance([cuda(k) for k in coords], cuda(S), cuda(bil), cuda(cjl))
+    y = host(op.forward(cuda(x)))
+    report("fused fine-grid SENSE forward (3, 128, 128) C=3", rel_l2(y, sense_forward(x, S, coords)), TOL)
