@@ -56,6 +56,8 @@ SIGNATURES = {
     "hasty_nufft_coord_grad": (c_int, [c_vp, c_vp, c_vp, c_vp]),
     "hasty_nufft_destroy": (c_int, [c_vp]),
     "hasty_nufft_info": (c_int, [c_vp, P_int, P_dbl, P_i64, P_i32, P_i64]),
+    "hasty_nufft_default_opts": (c_int, [ctypes.POINTER(NufftOpts)]),
+    "hasty_nufft_kernel_width": (c_int, [c_dbl, c_dbl, c_int, c_int, P_int]),
     "hasty_toeplitz_kernel": (c_int, [c_int, P_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_vp]),
     "hasty_toeplitz_multiply": (c_int, [c_int, P_i64, c_i64, c_vp, c_vp, c_vp]),
     "hasty_transform_toeplitz_kernel": (c_int, [c_int, P_i64, c_vp, c_vp]),

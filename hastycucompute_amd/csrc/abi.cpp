@@ -5,6 +5,7 @@
 #include <memory>
 #include <string>
 
+#include "kb.hpp"
 #include "nufft.hpp"
 
 #include <map>
@@ -137,6 +138,25 @@ int hasty_sense_adjoint(hasty_nufft_plan plan, int ncoils, int L, const void* kd
         HASTY_REQUIRE(plan != nullptr, "plan is NULL");
         sense_offres_adjoint(reinterpret_cast<NufftPlan*>(plan), ncoils, L, kdata, smaps, bil, cjl,
                              images, combine ? 1 : 0);
+    });
+}
+
+int hasty_nufft_default_opts(hasty_nufft_opts* opts) {
+    return guarded([&] {
+        HASTY_REQUIRE(opts != nullptr, "opts is NULL");
+        *opts = hasty_nufft_opts{};
+        opts->upsamp = 2.0;
+        opts->method = HASTY_METHOD_DEFAULT;
+    });
+}
+
+int hasty_nufft_kernel_width(double tol, double upsamp, int dim, int precision, int* width) {
+    return guarded([&] {
+        HASTY_REQUIRE(width != nullptr, "width is NULL");
+        HASTY_REQUIRE(tol > 0 && tol < 1, "tol must be in (0, 1)");
+        HASTY_REQUIRE(dim == 2 || dim == 3, "nufft dim must be 2 or 3");
+        HASTY_REQUIRE(precision == HASTY_F32 || precision == HASTY_F64, "precision must be HASTY_F32 or HASTY_F64");
+        *width = kb_width_for(tol, upsamp > 0 ? upsamp : 2.0, dim, precision == HASTY_F64);
     });
 }
 

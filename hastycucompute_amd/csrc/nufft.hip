@@ -1,6 +1,7 @@
 // Type-1 / type-2 NUFFT for gfx950: bin sort, LDS-tiled spread and interp,
 // pruned Stockham FFT passes with fused deapodisation.
 #include <algorithm>
+#include <cstdio>
 #include <cstdlib>
 #include <memory>
 #include <type_traits>
@@ -508,10 +509,15 @@ NufftPlan* nufft_makeplan(int type, int dim, const int64_t* nmodes, int sign, in
     p->tol = tol;
     p->sigma = (opts && opts->upsamp > 0) ? opts->upsamp : 2.0;
     HASTY_REQUIRE(p->sigma >= 1.25, "upsampling factor must be >= 1.25");
-    p->w = kb_width_for_tol(tol);
-    HASTY_REQUIRE(p->w <= (f64 ? kMaxWidthF64 : kMaxWidthF32),
-                  f64 ? "tolerance below 1e-13 is not supported"
-                      : "complex64 plans support tolerances down to 1e-8 (kernel width 9)");
+    p->w = kb_width_for(tol, p->sigma, dim, f64);
+    if (p->w > (f64 ? kMaxWidthF64 : kMaxWidthF32)) {
+        char msg[200];
+        std::snprintf(msg, sizeof msg,
+                      "tolerance %.3g at upsampling %.3g needs a kernel wider than the %d cells "
+                      "%s plans support",
+                      tol, p->sigma, f64 ? kMaxWidthF64 : kMaxWidthF32, f64 ? "complex128" : "complex64");
+        throw Error(HASTY_ERR_INVALID, msg);
+    }
     p->beta = kb_beta(p->w, p->sigma);
     p->stream = stream;
     p->device = current_device();
@@ -535,7 +541,10 @@ NufftPlan* nufft_makeplan(int type, int dim, const int64_t* nmodes, int sign, in
         p->ntile[s] = int(p->nf[s] / p->T[s]);
     }
     p->ntiles = int64_t(p->ntile[0]) * p->ntile[1] * p->ntile[2];
-    const int method = opts ? opts->method : 3;
+    const int method = opts ? opts->method : HASTY_METHOD_DEFAULT;
+    HASTY_REQUIRE(method >= HASTY_METHOD_GM_NO_SORT && method <= HASTY_METHOD_DEFAULT,
+                  "Invalid nufft_cuda_method");
+    const bool gm_method = method == HASTY_METHOD_GM_NO_SORT || method == HASTY_METHOD_GM_SORT;
     if (type == 2) {
         // one transform per block by default: a second staged tile halves
         // the occupancy, which costs more than re-reading the point data
@@ -543,7 +552,7 @@ NufftPlan* nufft_makeplan(int type, int dim, const int64_t* nmodes, int sign, in
         group = std::min(std::min(group, kMaxGroupInterp), ntransf);
         while (group > 1 && interp_lds_bytes(p.get(), group) > kLdsMax) --group;
         p->group = group;
-        p->gm = (method == 0 || method == 1) || interp_lds_bytes(p.get(), 1) > kLdsMax;
+        p->gm = gm_method || interp_lds_bytes(p.get(), 1) > kLdsMax;
     } else {
         // transforms per spreading block: a power of two covering the batch
         // (register accumulators per lane), capped at 32 (16 for complex128)
@@ -552,7 +561,7 @@ NufftPlan* nufft_makeplan(int type, int dim, const int64_t* nmodes, int sign, in
         int group = 1;
         while (group < std::min(want, cap)) group *= 2;
         p->group = group;
-        p->gm = (method == 0 || method == 1);
+        p->gm = gm_method;
     }
     p->max_batch = (opts && opts->max_batch > 0) ? std::min(opts->max_batch, ntransf) : ntransf;
     p->max_sub = (opts && opts->max_subproblem > 0) ? opts->max_subproblem : (type == 2 ? 8192 : 2048);

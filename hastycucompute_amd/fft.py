@@ -411,6 +411,13 @@ def toeplitz_scratch_bytes(shape: Sequence[int], batch: int) -> int:
     return int(r)
 
 
-def kb_width(tol: float) -> int:
-    """Kernel width the plans use for a tolerance."""
-    return int(min(16, max(2, math.ceil(-math.log10(tol / 10.0)))))
+def kb_width(tol: float, upsamp: float = 2.0, dim: int = 3, dtype: torch.dtype = torch.float32) -> int:
+    """Kernel width a plan of real type ``dtype`` uses for a tolerance:
+    ``ceil(-log10(tol / 10))`` at upsampling 2, else the smallest width whose
+    estimated error (window aliasing plus the rounding floor of the plan's
+    arithmetic) is <= tol; 17 when none up to 16 is (``kb_width_for`` in
+    ``csrc/kb.hpp``).  A live plan reports its width in ``nufft_plan.info()``."""
+    w = ctypes.c_int()
+    check(lib().hasty_nufft_kernel_width(float(tol), float(upsamp), int(dim),
+                                         1 if dtype == torch.float64 else 0, ctypes.byref(w)))
+    return int(w.value)

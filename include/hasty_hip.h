@@ -36,8 +36,22 @@ extern "C" {
 /* nufft_type (nufft.cppm:42-48) */
 #define HASTY_TYPE_1 1        /* BACKWARD: nonuniform -> modes (spread)  */
 #define HASTY_TYPE_2 2        /* FORWARD:  modes -> nonuniform (interp)  */
-/* nufft_method_cuda (nufft.cppm:65-70) is accepted for API parity; this
- * library always uses its bin-sorted LDS-tiled method.                    */
+/* nufft_method_cuda (nufft.cppm:65-70; the reference maps it onto cufinufft's
+ * gpu_method / gpu_sort at nufft.cppm:118-139).  The values are the
+ * reference's, so 0 is GM_NO_SORT, not "auto": a caller that zero-fills
+ * hasty_nufft_opts must set `method` (or start from hasty_nufft_default_opts).
+ *   GM_NO_SORT, GM_SORT: global-memory kernels, one thread per point over the
+ *     bin-sorted points (the sort also holds the precomputed window weights, so
+ *     GM_NO_SORT runs the same kernels as GM_SORT).  Type 1 adds into the fine
+ *     grid with FP32 / FP64 global atomics, so its sums are not in a fixed
+ *     order and its results are not bitwise reproducible.
+ *   SM, DEFAULT: the LDS-tiled kernels (type 2 stages a padded tile per block;
+ *     type 1 gathers each tile's visit list with register accumulators, no
+ *     atomics, bitwise reproducible).                                       */
+#define HASTY_METHOD_GM_NO_SORT 0
+#define HASTY_METHOD_GM_SORT 1
+#define HASTY_METHOD_SM 2
+#define HASTY_METHOD_DEFAULT 3
 /* ToeplitzMultType / ToeplitzAccumulateType (toeplitz.cppm:47-56) */
 #define HASTY_MULT_NONE 0
 #define HASTY_MULT 1
@@ -56,7 +70,7 @@ typedef struct hasty_normal_op_s* hasty_normal_op;
  * nufft_opts (nufft.cppm:78-95) plus this library's tiling knobs. */
 typedef struct hasty_nufft_opts {
     double upsamp;        /* oversampling sigma; <= 0 -> 2.0 (nufft_upsamp_cuda)  */
-    int32_t method;       /* accepted, ignored (nufft_method_cuda)                */
+    int32_t method;       /* HASTY_METHOD_* (nufft_method_cuda); 0 = GM_NO_SORT   */
     int32_t tile[3];      /* fine-grid tile per axis (slots z,y,x); 0 -> auto     */
     int32_t coil_group;   /* transforms per spread/interp block; 0 -> auto        */
     int32_t max_batch;    /* transforms per internal batch; 0 -> all              */
@@ -102,6 +116,18 @@ int hasty_nufft_set_stream(hasty_nufft_plan plan, void* stream);
  * with the opposite sign: grad_k = out(h = dL/df, w = c). */
 int hasty_nufft_coord_grad(hasty_nufft_plan plan, const void* h, const void* w, void* out);
 int hasty_nufft_destroy(hasty_nufft_plan plan);
+/* cufinufft_default_opts (called at nufft.cppm:117): upsamp 2, method
+ * DEFAULT, every tiling knob 0 (auto). */
+int hasty_nufft_default_opts(hasty_nufft_opts* opts);
+/* Kernel width a plan of this precision takes for (tol, upsamp, dim), without
+ * making one.  upsamp 2 (or <= 0): w = ceil(-log10(tol / 10)).  Other
+ * upsampling: the smallest w whose estimated relative L2 error (aliasing of
+ * the window plus the rounding of the plan's arithmetic, which the wide
+ * sigma-1.25 windows amplify) is <= tol: upsamp 1.25 gives w = 9 at 1e-5 for
+ * HASTY_F64 and w = 6 at 1e-3; HASTY_F32 bottoms out near 6e-5 there.
+ * 17 = no width up to 16 reaches tol.  makeplan rejects widths above 9
+ * (complex64) or 14 (complex128). */
+int hasty_nufft_kernel_width(double tol, double upsamp, int dim, int precision, int* width);
 /* introspection: width, beta, fine grid, tiles, device bytes held */
 int hasty_nufft_info(hasty_nufft_plan plan, int* width, double* beta, int64_t* nf3,
                      int32_t* tile3, int64_t* device_bytes);

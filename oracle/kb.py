@@ -18,7 +18,8 @@ Phi(a)  = w sinh(sqrt(beta^2 - (a w/2)^2)) / sqrt(beta^2 - (a w/2)^2)
           (the window's continuous Fourier transform at angular frequency a
            = 2 pi n / nf; the sin form when the root is imaginary)
 beta    = pi sqrt(w^2/sigma^2 (sigma - 1/2)^2 - 0.8)      (Beatty 2005, eq. 5)
-width   = ceil(-log10(tol / 10)), clamped to [2, 16]
+width   = ceil(-log10(tol / 10)) at sigma = 2, else the smallest width whose
+          aliasing estimate plus rounding floor is <= tol (``kernel_width``)
 
 Every step is FP64.  The fine grid is ``nf_a = next 2^a 3^b 5^c >= sigma N_a``.
 """
@@ -50,14 +51,58 @@ def _lib():
     return _LIB
 
 
-def kernel_width(tol: float) -> int:
-    """Window width in fine-grid cells for a requested tolerance."""
-    return int(min(16, max(2, math.ceil(-math.log10(tol / 10.0)))))
-
-
 def kernel_beta(width: int, sigma: float = 2.0) -> float:
     """Beatty et al. 2005 shape parameter for width ``w`` at oversampling ``sigma``."""
     return math.pi * math.sqrt(width * width / (sigma * sigma) * (sigma - 0.5) ** 2 - 0.8)
+
+
+def kernel_error_estimate(width: int, sigma: float = 2.0, dim: int = 3) -> float:
+    """Aliasing estimate of the relative L2 gridding error.
+
+    The window has compact support, so by Poisson summation the only error of
+    a mode at angular frequency alpha (|alpha| <= pi/sigma) is the replicas
+    ``Phi(alpha + 2 pi p)``, p != 0, relative to ``Phi(alpha)``.  RMS over 64
+    midpoints of [0, pi/sigma] with |p| <= 400, times sqrt(dim).
+    """
+    beta = kernel_beta(width, sigma)
+    al = np.pi / sigma * (np.arange(64) + 0.5) / 64
+    p = np.concatenate([np.arange(1, 401), -np.arange(1, 401)]).astype(np.float64)
+    with np.errstate(over="ignore", invalid="ignore"):
+        rep = kb_hat(al[:, None] + 2 * np.pi * p[None, :], width, beta)
+        h0 = kb_hat(al, width, beta)
+    e2 = (rep ** 2).sum(1) / h0 ** 2
+    return float(np.sqrt(dim * e2.mean()))
+
+
+def kernel_rounding_gain(width: int, sigma: float = 2.0) -> float:
+    """Per-axis growth of fine-grid rounding noise through the gridding steps:
+    RMS deapodisation factor sqrt(mean Phi(alpha)^-2) over [0, pi/sigma] times
+    the window's sqrt(int phi^2) (midpoint rule, 4096 points)."""
+    beta = kernel_beta(width, sigma)
+    al = np.pi / sigma * (np.arange(64) + 0.5) / 64
+    inv = np.sqrt(np.mean(kb_hat(al, width, beta) ** -2.0))
+    t = -0.5 * width + width * (np.arange(4096) + 0.5) / 4096
+    phi2 = np.sum(kb_window(t, width, beta) ** 2) * width / 4096
+    return float(inv * np.sqrt(phi2))
+
+
+def kernel_width(tol: float, sigma: float = 2.0, dim: int = 3, precision: str = "f32") -> int:
+    """Window width in fine-grid cells for a requested tolerance, for a plan
+    computing in ``precision`` ("f32" for complex64 data, "f64").
+
+    sigma = 2: FINUFFT's published rule ``ceil(-log10(tol / 10))``; any other
+    sigma: the smallest width whose ``kernel_error_estimate`` plus rounding
+    floor ``4 eps gain^dim`` (eps = 2^-24 or 2^-53) is <= tol.  17 means no
+    width up to 16 reaches tol.
+    """
+    if sigma == 2.0:
+        w = math.ceil(-math.log10(tol / 10.0))
+        return 2 if w < 2 else (17 if w > 16 else int(w))
+    eps = 2.0 ** -53 if precision == "f64" else 2.0 ** -24
+    for w in range(2, 17):
+        if kernel_error_estimate(w, sigma, dim) + 4 * eps * kernel_rounding_gain(w, sigma) ** dim <= tol:
+            return w
+    return 17
 
 
 def next_smooth(n: int) -> int:
@@ -120,10 +165,14 @@ class KBPlan:
     """
 
     def __init__(self, shape, tol: float = 1e-5, sigma: float = 2.0, dtype=np.complex128,
-                 workers: int | None = None):
+                 workers: int | None = None, width_for: str = "f32"):
+        """``width_for``: the precision whose width rule picks the window
+        (the library plan this one restates), independent of ``dtype``."""
         self.shape = tuple(int(s) for s in shape)
         self.d = len(self.shape)
-        self.w = kernel_width(tol)
+        self.w = kernel_width(tol, sigma, self.d, width_for)
+        if self.w > 16:
+            raise RuntimeError(f"tolerance {tol:g} at upsampling {sigma:g} needs a kernel wider than 16")
         self.beta = kernel_beta(self.w, sigma)
         self.nf = tuple(fine_size(n, sigma, self.w) for n in self.shape)
         self.dtype = np.dtype(dtype)

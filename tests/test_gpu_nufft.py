@@ -312,3 +312,89 @@ def test_fused_fine_grid_sense_forward_128_planes():
     op = mri.SenseOffresonance([cuda(k) for k in coords], cuda(S), cuda(bil), cuda(cjl))
     y = host(op.forward(cuda(x)))
     report("fused fine-grid SENSE forward (3, 128, 128) C=3", rel_l2(y, sense_forward(x, S, coords)), TOL)
+
+
+@pytest.mark.parametrize("tol,dtype", [(1e-3, torch.float32), (1e-4, torch.float32), (1e-5, torch.float64)])
+@pytest.mark.parametrize("shape,M,ntransf", [((24, 20, 16), 3000, 1), ((20, 16, 24), 2500, 3),
+                                             ((48, 40), 3000, 2)])
+def test_upsamp_1_25_meets_tolerance(tol, dtype, shape, M, ntransf):
+    """nufft_upsamp_cuda::UPSAMP_1_25 (``nufft.cppm:141-150``): on a 1.25x
+    fine grid the window is widened until its error estimate (aliasing plus
+    the plan's rounding floor) meets tol -- w = 6 at 1e-3, 7-8 at 1e-4 in
+    complex64, 9 at 1e-5 in complex128; the plan reports that width, the
+    oracle's rule gives the same one, and both transforms meet 2 tol against
+    the FP64 NDFT."""
+    from hastycucompute_amd import fft as hf
+    from oracle.kb import kernel_width, next_smooth
+    d = len(shape)
+    f64 = dtype == torch.float64
+    rdt, cdt = (np.float64, np.complex128) if f64 else (np.float32, np.complex64)
+    tdt = torch.complex128 if f64 else torch.complex64
+    rng = np.random.default_rng(int(tol * 1e6) + M + ntransf)
+    coords = [x.astype(rdt) for x in rand_coords(rng, d, M)]
+    f = crandn(rng, (ntransf,) + shape, cdt)
+    c = crandn(rng, (ntransf, M), cdt)
+    p1, p2 = _plans(shape, ntransf, tol=tol, dtype=dtype, upsamp=hf.nufft_upsamp_cuda.UPSAMP_1_25)
+    w = p2.info()["width"]
+    assert w == p1.info()["width"] == kernel_width(tol, 1.25, d, "f64" if f64 else "f32") \
+        == hf.kb_width(tol, 1.25, d, dtype)
+    assert w > hf.kb_width(tol, 2.0, d, dtype), "sigma 1.25 must widen the sigma-2 window"
+    nf = p2.info()["nf"][-d:]
+    assert tuple(nf) == tuple(next_smooth(max(int(np.ceil(1.25 * n)), 2 * w)) for n in shape)
+    gco = [cuda(x) for x in coords]
+    p1.setpts(gco)
+    p2.setpts(gco)
+    o2 = torch.empty((ntransf, M), dtype=tdt, device="cuda")
+    o1 = torch.empty((ntransf,) + shape, dtype=tdt, device="cuda")
+    p2.execute(cuda(f), o2)
+    p1.execute(cuda(c), o1)
+    bar = 2 * tol
+    tag = f"upsamp 1.25 {'c128' if f64 else 'c64'} tol={tol:g} w={w}"
+    report(f"{tag} type2 {shape} T={ntransf}", rel_l2(host(o2), ndft_type2(f, coords)), bar)
+    report(f"{tag} type1 {shape} T={ntransf}", rel_l2(host(o1), ndft_type1(c, coords, shape)), bar)
+
+
+def test_upsamp_1_25_unreachable_tolerance_raises():
+    """complex64 arithmetic on a 1.25x grid bottoms out near 6e-5 (the FP32
+    rounding of the fine-grid FFT grows with the wide window), so a
+    complex64 plan at 1e-5 is refused rather than missing tol silently; a
+    complex128 plan at 1e-5 is made."""
+    from hastycucompute_amd import fft as hf
+    with pytest.raises(RuntimeError, match="kernel wider"):
+        _plans((16, 16, 16), 1, tol=1e-5, upsamp=hf.nufft_upsamp_cuda.UPSAMP_1_25)
+    p1, p2 = _plans((16, 16, 16), 1, tol=1e-5, dtype=torch.float64, upsamp=hf.nufft_upsamp_cuda.UPSAMP_1_25)
+    assert p1.info()["width"] == p2.info()["width"] == 9
+
+
+@pytest.mark.parametrize("method", ["GM_NO_SORT", "GM_SORT"])
+@pytest.mark.parametrize("shape,M,ntransf", [((16, 20, 12), 3000, 1), ((20, 16, 24), 2500, 3),
+                                             ((64, 48), 4000, 1), ((33, 20), 2500, 2)])
+def test_global_memory_methods_vs_ndft(method, shape, M, ntransf):
+    """nufft_method_cuda::GM_NO_SORT / GM_SORT (``nufft.cppm:119-130``) run the
+    global-memory kernels (one thread per point; type 1 with global atomics):
+    both types, 2-D and 3-D, one and several transforms, vs the FP64 NDFT."""
+    from hastycucompute_amd import fft as hf
+    rng = np.random.default_rng(M + ntransf + len(shape))
+    coords = rand_coords(rng, len(shape), M)
+    f = crandn(rng, (ntransf,) + shape)
+    c = crandn(rng, (ntransf, M))
+    g1, g2 = _run(shape, coords, f, c, ntransf, method=hf.nufft_method_cuda[method])
+    report(f"{method} type2 {shape} T={ntransf}", rel_l2(g2, ndft_type2(f, coords)), TOL)
+    report(f"{method} type1 {shape} T={ntransf}", rel_l2(g1, ndft_type1(c, coords, shape)), TOL)
+
+
+def test_global_memory_method_kooshball_vs_kb_oracle():
+    """GM_SORT on a kooshball (points piled on the centre, so many atomics meet
+    on one cell) agrees with the LDS-tiled default to the KB oracle's bar."""
+    from hastycucompute_amd import fft as hf
+    from hastycucompute_amd import synth
+    shape, T = (32, 32, 32), 4
+    coords = synth.kooshball(256, 64)
+    M = len(coords[0])
+    rng = np.random.default_rng(22)
+    f = crandn(rng, (T,) + shape)
+    c = crandn(rng, (T, M))
+    g1, g2 = _run(shape, coords, f, c, T, method=hf.nufft_method_cuda.GM_SORT)
+    kb = KBPlan(shape, EPS).setpts(coords)
+    report("GM_SORT kooshball type2 vs KB oracle", rel_l2(g2, kb.type2(f)), 2e-6)
+    report("GM_SORT kooshball type1 vs KB oracle", rel_l2(g1, kb.type1(c)), 2e-6)

@@ -50,6 +50,19 @@ def test_kb_width_rule():
     assert hf.kb_width(1e-9) == 10
 
 
+def test_kb_width_library_matches_oracle():
+    """The library's width rule (host arithmetic, no GPU) is the oracle's, at
+    both upsampling factors, 2-D and 3-D; unreachable tolerances give 17."""
+    from oracle.kb import kernel_width
+    for sigma in (2.0, 1.25, 1.5):
+        for dim in (2, 3):
+            for tol in (1e-2, 1e-3, 3e-4, 1e-5, 1e-6, 1e-8, 1e-10):
+                assert hf.kb_width(tol, sigma, dim) == kernel_width(tol, sigma, dim), (tol, sigma, dim)
+                assert hf.kb_width(tol, sigma, dim, torch.float64) == kernel_width(tol, sigma, dim, "f64")
+    assert hf.kb_width(1e-14, 1.25, 3, torch.float64) == 17
+    assert hf.kb_width(1e-5, 1.25, 3) == 17 and hf.kb_width(1e-5, 1.25, 3, torch.float64) == 9
+
+
 def test_kooshball_geometry():
     kz, ky, kx = synth.kooshball(64, 32)
     for k in (kz, ky, kx):

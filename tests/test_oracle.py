@@ -8,7 +8,9 @@ would break:
 * ndft_type2 / ndft_type1 -- points on the Cartesian grid reduce to numpy's
   FFT with fftshift (pins sign, centred modes and axis pairing); type 1 is the
   adjoint of type 2; a point at k = 0 gives sum(f) / a constant image.
-* KBPlan -- against the NDFT (3-D, 2-D, batched), error shrinking with width.
+* KBPlan -- against the NDFT (3-D, 2-D, batched), error shrinking with width;
+  the width rule at upsampling 1.25 meets tol against the NDFT, and the
+  aliasing error estimate behind it is the measured error within 2x.
 * toeplitz_kernel / toeplitz_multiply -- the Toeplitz embedding theorem
   T(x) = (1/N^d) A^H A x against the NDFT pair; full Cartesian sampling gives
   T = identity; a one-hot kernel reduces the product to a circular shift.
@@ -22,7 +24,7 @@ import numpy as np
 import pytest
 
 from oracle import toeplitz as ot
-from oracle.kb import KBPlan, kernel_beta, kernel_width
+from oracle.kb import KBPlan, kernel_beta, kernel_width, next_smooth
 from oracle.ndft import ndft_type1, ndft_type2, rel_l2
 from oracle.solvers import conjugate_gradient, max_eig, sense_normal, sense_normal_direct
 
@@ -93,6 +95,76 @@ def test_kb_error_decreases_with_width():
     errs = [rel_l2(KBPlan(shape, tol).setpts(coords).type2(f), ref) for tol in (1e-3, 1e-5, 1e-7)]
     assert errs[0] > errs[1] > errs[2]
     assert kernel_width(1e-5) == 6 and abs(kernel_beta(6) - 13.8551) < 1e-3
+
+
+@pytest.mark.parametrize("sigma", [2.0, 1.25])
+def test_kb_error_estimate_tracks_ndft(sigma):
+    """The aliasing estimate that picks the width away from sigma = 2 is the
+    measured error vs the FP64 NDFT to within a factor 2, for every width from
+    the coarse to the 1e-9 range (a dropped replica side, a wrong sqrt(dim) or
+    a wrong alpha range moves it by more)."""
+    from oracle.kb import fine_size, deapod_factors, kernel_error_estimate
+    rng = np.random.default_rng(7)
+    shape, M = (24, 20, 16), 2500
+    coords = [rng.uniform(-np.pi, np.pi, M) for _ in shape]
+    f = crandn(rng, shape)
+    ref = ndft_type2(f, coords)
+    for w in range(3, 15, 2):
+        p = KBPlan(shape, 1e-3, sigma=sigma)
+        p.w, p.beta = w, kernel_beta(w, sigma)
+        p.nf = tuple(fine_size(n, sigma, w) for n in shape)
+        p.deapod = [deapod_factors(n, nf, w, p.beta) for n, nf in zip(shape, p.nf)]
+        err = rel_l2(p.setpts(coords).type2(f), ref)
+        est = kernel_error_estimate(w, sigma, 3)
+        assert 0.5 < est / err < 2.0, (sigma, w, est, err)
+
+
+def test_kb_width_rule_sigma_1_25():
+    """sigma = 2 keeps ceil(-log10(tol/10)); sigma = 1.25 takes the smallest
+    width whose estimate meets tol, and the oracle plan then meets tol vs the
+    FP64 NDFT where the sigma-2 width (w = 6) misses it by ~40x."""
+    assert [kernel_width(t) for t in (1e-3, 1e-5, 1e-6, 1e-9, 1e-13)] == [4, 6, 7, 10, 14]
+    assert kernel_width(1e-5, 1.25, 3, "f64") == 9 and kernel_width(1e-3, 1.25, 3) == 6
+    assert kernel_width(1e-5, 1.25, 2, "f64") == 9 and kernel_width(1e-3, 1.25, 2) == 6
+    assert kernel_width(1e-4, 1.25, 3, "f32") == 8 and kernel_width(1e-5, 1.25, 3, "f32") == 17
+    rng = np.random.default_rng(8)
+    shape, M = (20, 16, 24), 2000
+    coords = [rng.uniform(-np.pi, np.pi, M) for _ in shape]
+    f = crandn(rng, shape)
+    c = crandn(rng, M)
+    for tol in (1e-3, 1e-5):
+        p = KBPlan(shape, tol, sigma=1.25, width_for="f64").setpts(coords)
+        assert p.nf == tuple(next_smooth(max(int(np.ceil(1.25 * n)), 2 * p.w)) for n in shape)
+        assert rel_l2(p.type2(f), ndft_type2(f, coords)) <= tol
+        assert rel_l2(p.type1(c), ndft_type1(c, coords, shape)) <= tol
+
+
+def test_kb_rounding_floor_sigma_1_25():
+    """In complex64 arithmetic the sigma-1.25 windows amplify the FFT's
+    rounding: the oracle computing in complex64 misses 2e-5 at w = 9 while the
+    same plan in complex128 meets it; the rule's complex64 width at 1e-4
+    (w = 8) meets 1e-4 in complex64 arithmetic, and its floor term brackets
+    the measured complex64 excess within 4x."""
+    from oracle.kb import kernel_error_estimate, kernel_rounding_gain
+    rng = np.random.default_rng(9)
+    shape, M = (24, 20, 16), 2500
+    coords = [rng.uniform(-np.pi, np.pi, M).astype(np.float32).astype(np.float64) for _ in shape]
+    f = crandn(rng, shape)
+    c = crandn(rng, M)
+    ref2, ref1 = ndft_type2(f, coords), ndft_type1(c, coords, shape)
+    p64 = KBPlan(shape, 1e-5, sigma=1.25, width_for="f64").setpts(coords)
+    p32 = KBPlan(shape, 1e-5, sigma=1.25, width_for="f64", dtype=np.complex64).setpts(coords)
+    assert p64.w == p32.w == 9
+    e64 = rel_l2(p64.type2(f), ref2)
+    e32 = rel_l2(p32.type2(f.astype(np.complex64)), ref2)
+    assert e64 <= 1e-5 < 2e-5 < e32
+    floor = 4 * 2.0 ** -24 * kernel_rounding_gain(9, 1.25) ** 3
+    assert floor / 4 < e32 - e64 < 4 * floor
+    q = KBPlan(shape, 1e-4, sigma=1.25, dtype=np.complex64).setpts(coords)
+    assert q.w == 8
+    assert rel_l2(q.type2(f.astype(np.complex64)), ref2) <= 1e-4
+    assert rel_l2(q.type1(c.astype(np.complex64)), ref1) <= 1e-4
+    assert kernel_error_estimate(8, 1.25, 3) < 1e-4
 
 
 def test_toeplitz_embedding_theorem():
