@@ -75,6 +75,7 @@ SIGNATURES = {
     "hasty_normal_set_stream": (c_int, [c_vp, c_vp]),
     "hasty_normal_destroy": (c_int, [c_vp]),
     "hasty_dot": (c_int, [c_vp, c_vp, c_i64, c_vp, c_vp]),
+    "hasty_dot_noalloc": (c_int, [c_vp, c_vp, c_i64, c_vp, c_vp]),
     "hasty_cg_init": (c_int, [c_vp, c_vp, c_i64, c_dbl, c_vp, c_vp]),
     "hasty_cg_step": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp]),
     "hasty_cg_status": (c_int, [c_vp, P_dbl, P_int, c_vp]),

@@ -4,6 +4,8 @@
 // alpha = rz / Re<p,Ap>, x += alpha p, r -= alpha Ap, rz' = Re<r,r>,
 // p = r + (rz'/rz) p; stop once sqrt(rz) < tol) without a host sync per
 // iteration: a device flag turns the remaining launches into no-ops.
+#include <mutex>
+
 #include "common.hpp"
 
 namespace hasty {
@@ -223,6 +225,35 @@ static unsigned red_blocks(int64_t n) {
     return unsigned(std::max<int64_t>(1, std::min<int64_t>(kRedBlocks, ceil_div(n, kRedThreads))));
 }
 
+// Scratch of hasty_dot_noalloc: the per-block FP64 partial sums (Re block,
+// then Im block) of one dot product, 2 x kRedBlocks doubles = 8 KiB on every
+// device the library runs on, held until the process ends.  Every partial a
+// launch reads is written by the same launch, so nothing needs clearing.
+// Host code never passes `scratch` itself (its host-side name is not a device
+// address): it passes the per-device address from hipGetSymbolAddress.
+constexpr int kDotScratchDoubles = 2 * kRedBlocks;
+static __device__ __align__(256) double scratch[kDotScratchDoubles];
+
+constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per
+// device (on that device's first hasty_dot_noalloc call).
+static double* dot_scratch_on_current_device() {
+    int dev = -1;
+    HASTY_CHECK_HIP(hipGetDevice(&dev));
+    HASTY_REQUIRE(dev >= 0 && dev < kMaxScratchDevices, "device index out of range for the dot scratch");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* ptr = nullptr;
+        HASTY_CHECK_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        HASTY_REQUIRE(ptr != nullptr, "hipGetSymbolAddress returned NULL for the dot scratch");
+        g_scratch_ptr[dev] = static_cast<double*>(ptr);
+    }
+    return g_scratch_ptr[dev];
+}
+
 }  // namespace hasty
 
 using namespace hasty;
@@ -243,6 +274,24 @@ int hasty_dot(const void* a, const void* b, int64_t n, double* out, void* stream
         hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(kRedThreads), 0, s, part, int(nb), out);
         HASTY_CHECK_HIP(hipGetLastError());
         HASTY_CHECK_HIP(hipFreeAsync(part, s));
+    });
+}
+
+int hasty_dot_noalloc(const void* a, const void* b, int64_t n, double* out, void* stream) {
+    return guarded([&] {
+        HASTY_REQUIRE(n >= 0 && out, "invalid arguments to hasty_dot_noalloc");
+        HASTY_REQUIRE(n == 0 || (a && b), "NULL vector");
+        const unsigned nb = red_blocks(n);
+        // the partials of nb blocks must fit the static scratch (nb <= kRedBlocks for any n)
+        HASTY_REQUIRE(2 * int64_t(nb) <= kDotScratchDoubles,
+                      "hasty_dot_noalloc: partial sums exceed the 8 KiB static scratch");
+        double* part = dot_scratch_on_current_device();
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        hipLaunchKernelGGL(dot_partial_kernel, dim3(nb), dim3(kRedThreads), 0, s,
+                           static_cast<const float2*>(a), static_cast<const float2*>(b), n, part,
+                           static_cast<const double*>(nullptr));
+        hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(kRedThreads), 0, s, part, int(nb), out);
+        HASTY_CHECK_HIP(hipGetLastError());
     });
 }
 

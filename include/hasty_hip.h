@@ -219,6 +219,15 @@ int hasty_sense_adjoint(hasty_nufft_plan plan, int ncoils, int L, const void* kd
 #define HASTY_CG_STATE_DOUBLES 1040
 /* out[0] = Re sum conj(a) b, out[1] = Im (device doubles), n complex elements */
 int hasty_dot(const void* a, const void* b, int64_t n, double* out, void* stream);
+/* hasty_dot without allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same result as
+ * hasty_dot, bit for bit (same kernels and summation order); its partial sums
+ * live in a static 8 KiB device array per GPU (2 x 512 doubles, enough for
+ * any n) instead of a hipMallocAsync buffer, and the call allocates nothing.
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (different streams, or host threads without ordering) race. */
+int hasty_dot_noalloc(const void* a, const void* b, int64_t n, double* out, void* stream);
 /* state <- {rz = ||r||^2, done = sqrt(rz) < tol}; p <- r */
 int hasty_cg_init(void* r, void* p, int64_t n, double tol, double* state, void* stream);
 /* one CG update given Ap: pAp, alpha, x += alpha p, r -= alpha Ap, rz', p = r + (rz'/rz) p.
