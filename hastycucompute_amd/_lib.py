@@ -79,6 +79,9 @@ SIGNATURES = {
     "hasty_cg_init": (c_int, [c_vp, c_vp, c_i64, c_dbl, c_vp, c_vp]),
     "hasty_cg_step": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp]),
     "hasty_cg_status": (c_int, [c_vp, P_dbl, P_int, c_vp]),
+    "hasty_pcg_init": (c_int, [c_vp, c_vp, c_vp, c_i64, c_dbl, c_vp, c_vp]),
+    "hasty_pcg_update": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp]),
+    "hasty_pcg_direction": (c_int, [c_vp, c_vp, c_vp, c_i64, c_int, c_vp, c_vp]),
     "hasty_scale_by_device": (c_int, [c_vp, c_i64, c_vp, c_int, c_vp]),
     "hasty_norm": (c_int, [c_vp, c_i64, c_vp, c_vp]),
     "hasty_axpy": (c_int, [ctypes.c_double, ctypes.c_double, c_vp, c_vp, c_i64, c_vp]),

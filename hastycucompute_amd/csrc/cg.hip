@@ -87,9 +87,11 @@ __global__ void cg_alpha_kernel(double* st) {
     }
 }
 
-// x += alpha p ; r -= alpha Ap ; partial ||r||^2
+// x += alpha p ; r -= alpha Ap ; partial ||r||^2 (the preconditioned form
+// takes Re<r, z> after the preconditioner instead: want_norm = 0)
 __global__ void __launch_bounds__(kRedThreads) cg_update_kernel(float2* x, float2* r, const float2* p,
-                                                                const float2* Ap, int64_t n, double* st) {
+                                                                const float2* Ap, int64_t n, double* st,
+                                                                int want_norm) {
     if (st[ST_DONE] != 0.0) return;
     __shared__ double sh[kRedThreads];
     const double alpha = st[ST_ALPHA];
@@ -105,8 +107,25 @@ __global__ void __launch_bounds__(kRedThreads) cg_update_kernel(float2* x, float
         r[i] = ri;
         acc += double(ri.x) * double(ri.x) + double(ri.y) * double(ri.y);
     }
+    if (!want_norm) return;
     acc = block_sum(acc, sh);
     if (threadIdx.x == 0) st[ST_PART + kRedBlocks + blockIdx.x] = acc;
+}
+
+// Preconditioned direction (cg.cppm:45-56): rz' = Re<r, z> from the Re
+// partials of dot_partial_kernel; beta = 0 on a restart (p = z), else rz'/rz.
+__global__ void pcg_beta_kernel(double* st, int restart) {
+    __shared__ double sh[kRedThreads];
+    if (st[ST_DONE] != 0.0) return;
+    double v = 0.0;
+    for (int i = threadIdx.x; i < kRedBlocks; i += blockDim.x) v += st[ST_PART + i];
+    v = block_sum(v, sh);
+    if (threadIdx.x == 0) {
+        st[ST_RZNEW] = v;
+        st[ST_BETA] = (restart || st[ST_RZ] == 0.0) ? 0.0 : v / st[ST_RZ];
+        st[ST_RZ] = v;
+        st[ST_ITER] += 1.0;
+    }
 }
 
 // rz' = sum ; beta = rz'/rz ; rz = rz' ; done = sqrt(rz) < tol
@@ -155,6 +174,21 @@ __global__ void __launch_bounds__(kRedThreads) copy_norm_kernel(const float2* r,
         const float2 ri = r[i];
         p[i] = ri;
         acc += double(ri.x) * double(ri.x) + double(ri.y) * double(ri.y);
+    }
+    acc = block_sum(acc, sh);
+    if (threadIdx.x == 0) st[ST_PART + kRedBlocks + blockIdx.x] = acc;
+}
+
+// p <- z ; partial Re<r, z> (the preconditioned start, cg.cppm:81-85)
+__global__ void __launch_bounds__(kRedThreads) copy_dot_kernel(const float2* r, const float2* z, float2* p,
+                                                               int64_t n, double* st) {
+    __shared__ double sh[kRedThreads];
+    double acc = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * blockDim.x) {
+        const float2 ri = r[i], zi = z[i];
+        p[i] = zi;
+        acc += double(ri.x) * double(zi.x) + double(ri.y) * double(zi.y);
     }
     acc = block_sum(acc, sh);
     if (threadIdx.x == 0) st[ST_PART + kRedBlocks + blockIdx.x] = acc;
@@ -355,10 +389,58 @@ int hasty_cg_step(void* x, void* r, void* p, const void* Ap, int64_t n, double* 
         hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(kRedThreads), 0, s, state);
         hipLaunchKernelGGL(cg_update_kernel, dim3(nb), dim3(kRedThreads), 0, s,
                            static_cast<float2*>(x), static_cast<float2*>(r),
-                           static_cast<const float2*>(p), static_cast<const float2*>(Ap), n, state);
+                           static_cast<const float2*>(p), static_cast<const float2*>(Ap), n, state, 1);
         hipLaunchKernelGGL(cg_beta_kernel, dim3(1), dim3(kRedThreads), 0, s, state);
         hipLaunchKernelGGL(cg_p_kernel, dim3(nb), dim3(kRedThreads), 0, s, static_cast<float2*>(p),
                            static_cast<const float2*>(r), n, static_cast<const double*>(state));
+        hipLaunchKernelGGL(cg_finish_kernel, dim3(1), dim3(1), 0, s, state);
+        HASTY_CHECK_HIP(hipGetLastError());
+    });
+}
+
+int hasty_pcg_init(const void* r, const void* z, void* p, int64_t n, double tol, double* state,
+                   void* stream) {
+    return guarded([&] {
+        HASTY_REQUIRE(r && z && p && state && n >= 0, "invalid arguments to hasty_pcg_init");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        hipLaunchKernelGGL(zero_doubles_kernel, dim3(4), dim3(256), 0, s, state, HASTY_CG_STATE_DOUBLES);
+        hipLaunchKernelGGL(copy_dot_kernel, dim3(red_blocks(n)), dim3(kRedThreads), 0, s,
+                           static_cast<const float2*>(r), static_cast<const float2*>(z), static_cast<float2*>(p),
+                           n, state);
+        hipLaunchKernelGGL(cg_init_finish_kernel, dim3(1), dim3(kRedThreads), 0, s, state, tol);
+        HASTY_CHECK_HIP(hipGetLastError());
+    });
+}
+
+int hasty_pcg_update(void* x, void* r, const void* p, const void* Ap, int64_t n, double* state,
+                     void* stream) {
+    return guarded([&] {
+        HASTY_REQUIRE(x && r && p && Ap && state && n >= 0, "invalid arguments to hasty_pcg_update");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        const unsigned nb = red_blocks(n);
+        hipLaunchKernelGGL(dot_partial_kernel, dim3(nb), dim3(kRedThreads), 0, s,
+                           static_cast<const float2*>(p), static_cast<const float2*>(Ap), n,
+                           state + ST_PART, static_cast<const double*>(state + ST_DONE));
+        hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(kRedThreads), 0, s, state);
+        hipLaunchKernelGGL(cg_update_kernel, dim3(nb), dim3(kRedThreads), 0, s,
+                           static_cast<float2*>(x), static_cast<float2*>(r),
+                           static_cast<const float2*>(p), static_cast<const float2*>(Ap), n, state, 0);
+        HASTY_CHECK_HIP(hipGetLastError());
+    });
+}
+
+int hasty_pcg_direction(const void* r, const void* z, void* p, int64_t n, int restart, double* state,
+                        void* stream) {
+    return guarded([&] {
+        HASTY_REQUIRE(r && z && p && state && n >= 0, "invalid arguments to hasty_pcg_direction");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        const unsigned nb = red_blocks(n);
+        hipLaunchKernelGGL(dot_partial_kernel, dim3(nb), dim3(kRedThreads), 0, s,
+                           static_cast<const float2*>(r), static_cast<const float2*>(z), n,
+                           state + ST_PART, static_cast<const double*>(state + ST_DONE));
+        hipLaunchKernelGGL(pcg_beta_kernel, dim3(1), dim3(kRedThreads), 0, s, state, restart ? 1 : 0);
+        hipLaunchKernelGGL(cg_p_kernel, dim3(nb), dim3(kRedThreads), 0, s, static_cast<float2*>(p),
+                           static_cast<const float2*>(z), n, static_cast<const double*>(state));
         hipLaunchKernelGGL(cg_finish_kernel, dim3(1), dim3(1), 0, s, state);
         HASTY_CHECK_HIP(hipGetLastError());
     });

@@ -15,6 +15,7 @@
 //   sense_forward_offresonance                 cpp/corelib/lib/mri/forward.cppm:36-165
 //   sense_adjoint_offresonance                 cpp/corelib/lib/mri/adjoint.cppm:33-171
 //   conjugate_gradient(sptr<Op>, max_iter, tol) cpp/corelib/lib/op/cg.cppm:122-184
+//   conjugate_gradient(sptr<Op>, sptr<PrecondOp>, inner, outer, tol)   cg.cppm:16-119
 //   max_eig(A, x0, max_iter)                   cpp/corelib/lib/op/alg.cppm:13-30
 //
 // The reference's tensor<D,TT,RANK> wraps a torch tensor; here tensor<cuda_t,..>
@@ -1165,6 +1166,64 @@ template <class Op> auto conjugate_gradient(sptr<Op> A, i32 max_iter = 0, double
             detail::check(hasty_cg_step(x.mutable_data_ptr(), r.mutable_data_ptr(), p.mutable_data_ptr(),
                                         Ap.data_ptr(), n, state.mutable_data_ptr(), nullptr));
             detail::check(hasty_cg_status(state.data_ptr(), &resid, &done, nullptr));
+        }
+        return x;
+    };
+}
+
+// Preconditioned CG with restart callbacks (cg.cppm:16-119).  Returns
+// pcg(x, b, should_restart_callback, after_restart_callback) -> x:
+//   r = b - A x, z = P r, p = z, rz = Re<r, z>; each outer iteration runs up
+//   to max_inner_iter steps while sqrt(rz) >= tol: alpha = rz / Re<p, Ap>,
+//   x += alpha p, r -= alpha Ap, z = P r, rz' = Re<r, z>, p = z when
+//   should_restart_callback(x) (which also ends the inner loop) else
+//   z + (rz'/rz) p; then after_restart_callback(x).
+// FP64 scalars on the device (hasty_pcg_*); the residual is read on the host
+// once per step, as in the reference.  The reference's requires clause names
+// output_tensor_rank_t, which its operators do not define (cg.cppm:17-19);
+// the same constraints are checked here on the members that exist.
+template <class Op, class PrecondOp>
+auto conjugate_gradient(sptr<Op> A, sptr<PrecondOp> P, i32 max_inner_iter, i32 max_outer_iter,
+                        double tol = 1e-6) {
+    static_assert(std::is_same_v<typename Op::device_type_t, typename PrecondOp::device_type_t>,
+                  "A and P must run on the same device type");
+    static_assert(std::is_same_v<typename Op::output_tensor_type_t, typename PrecondOp::input_tensor_type_t>,
+                  "P must take A's output element type");
+    static_assert(Op::output_rank_t() == PrecondOp::input_rank_t(), "P must take A's output rank");
+    constexpr size_t R = Op::input_rank_t();
+    using T = tensor<cuda_t, c64_t, R>;
+    return [A = std::move(A), P = std::move(P), max_inner_iter, max_outer_iter, tol](
+               T& x, const T& b, std::optional<std::function<bool(T&)>> should_restart_callback = std::nullopt,
+               std::optional<std::function<void(T&)>> after_restart_callback = std::nullopt) -> T& {
+        const int64_t n = x.numel();
+        auto state = tensor<cuda_t, double, 1>::zeros({HASTY_CG_STATE_DOUBLES});
+        T r = b.clone();
+        {
+            T Ax = (*A)(x);
+            detail::check(hasty_axpy(-1.0, 0.0, Ax.data_ptr(), r.mutable_data_ptr(), n, nullptr));
+        }
+        T z = T::empty(x.shape_arr());
+        T p = T::empty(x.shape_arr());
+        T Ap = T::empty(x.shape_arr());
+        P->apply(r, z);
+        detail::check(hasty_pcg_init(r.data_ptr(), z.data_ptr(), p.mutable_data_ptr(), n, tol,
+                                     state.mutable_data_ptr(), nullptr));
+        double resid = 0.0;
+        int done = 0;
+        detail::check(hasty_cg_status(state.data_ptr(), &resid, &done, nullptr));
+        for (i32 outer = 0; outer < max_outer_iter; ++outer) {
+            for (i32 inner = 0; inner < max_inner_iter && !done; ++inner) {
+                A->apply(p, Ap);
+                detail::check(hasty_pcg_update(x.mutable_data_ptr(), r.mutable_data_ptr(), p.data_ptr(),
+                                               Ap.data_ptr(), n, state.mutable_data_ptr(), nullptr));
+                P->apply(r, z);
+                const bool restart = should_restart_callback.has_value() && (*should_restart_callback)(x);
+                detail::check(hasty_pcg_direction(r.data_ptr(), z.data_ptr(), p.mutable_data_ptr(), n,
+                                                  restart ? 1 : 0, state.mutable_data_ptr(), nullptr));
+                detail::check(hasty_cg_status(state.data_ptr(), &resid, &done, nullptr));
+                if (restart) break;
+            }
+            if (after_restart_callback.has_value()) (*after_restart_callback)(x);
         }
         return x;
     };

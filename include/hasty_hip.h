@@ -234,6 +234,20 @@ int hasty_cg_init(void* r, void* p, int64_t n, double tol, double* state, void* 
  * No-op once state.done is set.  */
 int hasty_cg_step(void* x, void* r, void* p, const void* Ap, int64_t n, double* state,
                   void* stream);
+/* Preconditioned CG with restarts (cg.cppm:16-119), as three calls around the
+ * caller's A and P applies: z = P(r) comes from the caller between calls.
+ *   pcg_init:      rz = Re<r, z>, p <- z, done = sqrt(rz) < tol
+ *   pcg_update:    pAp = Re<p, Ap>, alpha = rz / pAp, x += alpha p, r -= alpha Ap
+ *   pcg_direction: rz' = Re<r, z>; p <- z if restart, else z + (rz'/rz) p;
+ *                  rz = rz', done = sqrt(rz) < tol
+ * FP64 reductions and scalars on the device; update / direction are no-ops
+ * once done is set.  state as for hasty_cg_*; hasty_cg_status reads it. */
+int hasty_pcg_init(const void* r, const void* z, void* p, int64_t n, double tol, double* state,
+                   void* stream);
+int hasty_pcg_update(void* x, void* r, const void* p, const void* Ap, int64_t n, double* state,
+                     void* stream);
+int hasty_pcg_direction(const void* r, const void* z, void* p, int64_t n, int restart, double* state,
+                        void* stream);
 /* host read of residual and convergence flag (synchronises the stream) */
 int hasty_cg_status(const double* state, double* resid, int* done, void* stream);
 /* y <- a * x (a real FP64 scalar from device memory, scaled by 1/a_ptr[0] if invert) */

@@ -11,6 +11,12 @@
                         x += alpha p, r -= alpha Ap, rz' = Re<r,r>,
                         p = r + (rz'/rz) p; stop when sqrt(rz) < tol.
                         Inner products and scalars in FP64.
+* ``preconditioned_conjugate_gradient`` -- ``cg.cppm:16-119``: r = b - A x,
+                        z = P r, p = z, rz = Re<r,z>; inner iterations (while
+                        sqrt(rz) >= tol): alpha = rz / Re<p,Ap>, x += alpha p,
+                        r -= alpha Ap, z = P r, rz' = Re<r,z>, p = z on a
+                        restart else z + (rz'/rz) p; a restart ends the inner
+                        loop; after_restart(x) after every outer iteration.
 * ``max_eig``        -- ``alg.cppm:13-30``: y = A x, lambda = ||y||, x = y / lambda.
 """
 from __future__ import annotations
@@ -72,6 +78,36 @@ def conjugate_gradient(A, b, x0, max_iter: int, tol: float = 0.0, history=None):
         resid = np.sqrt(rz)
         if history is not None:
             history.append(resid)
+    return x
+
+
+def preconditioned_conjugate_gradient(A, P, b, x0, max_inner_iter: int, max_outer_iter: int,
+                                      tol: float = 1e-6, should_restart=None, after_restart=None):
+    x = np.array(x0, np.complex128, copy=True)
+    b = np.asarray(b, np.complex128)
+    r = b - A(x)
+    z = P(r)
+    p = z.copy()
+    rz = float(np.vdot(r, z).real)
+    resid = np.sqrt(rz)
+    for _ in range(max_outer_iter):
+        for _ in range(max_inner_iter):
+            if resid < tol:
+                break
+            Ap = A(p)
+            alpha = rz / float(np.vdot(p, Ap).real)
+            x = x + alpha * p
+            r = r - alpha * Ap
+            z = P(r)
+            restart = bool(should_restart(x)) if should_restart is not None else False
+            rznew = float(np.vdot(r, z).real)
+            p = z.copy() if restart else z + (rznew / rz) * p
+            rz = rznew
+            resid = np.sqrt(rz)
+            if restart:
+                break
+        if after_restart is not None:
+            after_restart(x)
     return x
 
 

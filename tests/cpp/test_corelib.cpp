@@ -55,6 +55,20 @@ Problem make_problem() {
     return p;
 }
 
+// P = a I on the device, with the operator members the PCG overload checks
+struct ScaleOp {
+    using device_type_t = cuda_t;
+    using input_tensor_type_t = c64_t;
+    using output_tensor_type_t = c64_t;
+    static constexpr size_t input_rank_t() { return 3; }
+    static constexpr size_t output_rank_t() { return 3; }
+    double a;
+    void apply(const tensor<cuda_t, c64_t, 3>& x, tensor<cuda_t, c64_t, 3>& out) const {
+        detail::check_hip(hipMemset(out.mutable_data_ptr(), 0, size_t(out.numel()) * sizeof(c64_t)), "hipMemset");
+        detail::check(hasty_axpy(a, 0.0, x.data_ptr(), out.mutable_data_ptr(), x.numel(), nullptr));
+    }
+};
+
 // c_j = sum_n f[n] exp(sign i k_j . n), centred modes, axis a <-> k[a]
 std::vector<std::complex<double>> ndft2(const Problem& p, const std::vector<c64_t>& f, int sign) {
     std::vector<std::complex<double>> c(size_t(p.M));
@@ -160,12 +174,19 @@ int main() {
         report("nufft_plan<TYPE_1> coil 0 vs NDFT", rel_l2(got, ref1), 2e-5);
     }
 
-    // the per-coil host path gives the batched result
+    // the per-coil host path: each coil against the FP64 NDFT of its data
+    std::vector<std::vector<std::complex<double>>> ref1(size_t(p.C));
+    for (int c = 0; c < p.C; ++c)
+        ref1[size_t(c)] = ndft1(p, std::vector<std::complex<double>>(hk.begin() + c * p.M, hk.begin() + (c + 1) * p.M),
+                                +1);
     {
         tensor<cpu_t, c64_t, 2> hin({p.C, p.M});
         hin.vec() = hk;
         auto hout = nufft_backward_cpu_over_cuda<c64_t, f32_t, 3>(p.n, hin, coords);
-        report("nufft_backward_cpu_over_cuda vs batched plan", rel_l2(hout.vec(), hadj), 2e-5);
+        const auto hv = hout.vec();
+        for (int c = 0; c < p.C; ++c)
+            report(("nufft_backward_cpu_over_cuda coil " + std::to_string(c) + " vs NDFT").c_str(),
+                   rel_l2(std::vector<c64_t>(hv.begin() + c * I, hv.begin() + (c + 1) * I), ref1[size_t(c)]), 2e-5);
     }
 
     // Toeplitz kernel + SENSE normal operator vs NDFT: sum_c conj(S_c) (1/N^3) A^H A (S_c x)
@@ -361,9 +382,19 @@ int main() {
         bool cleared = true;
         for (const auto& s : pool->storages()) cleared &= !s.exist("nufft_plan");
         const auto xp = sense_adjoint_offresonance<cuda_t, c64_t, f32_t, 3>(ccoords, ckd, cmaps, cbil, ccjl, iface);
-        report("sense_forward_offresonance on a 2-worker pool vs batched", rel_l2(yp.vec(), yc.vec()), 1e-6);
+        // every coil of the pool forms against the FP64 NDFT: y_c = A (S_c x) and
+        // x_c = conj(S_c) A^H y_c (one segment of unit factors)
+        const auto ypv = yp.vec(), xpv = xp.vec();
+        for (int c = 0; c < p.C; ++c) {
+            report(("sense_forward_offresonance 2-worker pool coil " + std::to_string(c) + " vs NDFT").c_str(),
+                   rel_l2(std::vector<c64_t>(ypv.begin() + c * p.M, ypv.begin() + (c + 1) * p.M), ref2[size_t(c)]),
+                   2e-5);
+            std::vector<std::complex<double>> xref = ref1[size_t(c)];
+            for (int64_t i = 0; i < I; ++i) xref[size_t(i)] *= std::conj(std::complex<double>(p.S[size_t(c * I + i)]));
+            report(("sense_adjoint_offresonance 2-worker pool coil " + std::to_string(c) + " vs NDFT").c_str(),
+                   rel_l2(std::vector<c64_t>(xpv.begin() + c * I, xpv.begin() + (c + 1) * I), xref), 2e-5);
+        }
         report("sense_forward_offresonance with the kept plan (0 = bitwise)", yp2.vec() == yp.vec() ? 0.0 : 1.0, 0.5);
-        report("sense_adjoint_offresonance on a 2-worker pool vs batched", rel_l2(xp.vec(), xc.vec()), 1e-6);
         report("pool keeps the plan, then free_nufft_plan clears it (0 = yes)", kept && cleared ? 0.0 : 1.0, 0.5);
         expect_throw("pool worker without a device_idx", "Storage does not contain key: device_idx", [&] {
             std::vector<threading::storage> bare(1);
@@ -382,6 +413,22 @@ int main() {
         const auto Ax = (*op)(x).to_host();
         const auto hb = b.to_host();
         report("conjugate_gradient: ||A x - b|| / ||b||", rel_l2(Ax, hb), 1e-3);
+        // preconditioned CG (cg.cppm:16-119) with P = 0.5 I takes the plain CG
+        // steps (z = r / 2 rescales alpha by 2 and p by 1/2); a restart on the
+        // 4th step and the after-restart callback once per outer iteration
+        auto xp = tensor<cuda_t, c64_t, 3>::zeros(p.n);
+        auto pcg = conjugate_gradient(op, std::make_shared<ScaleOp>(ScaleOp{0.5}), 10, 3, 0.0);
+        int steps = 0, outers = 0;
+        auto xc = tensor<cuda_t, c64_t, 3>::zeros(p.n);
+        conjugate_gradient(op, 10, 0.0)(xc, b);
+        auto x10 = tensor<cuda_t, c64_t, 3>::zeros(p.n);
+        conjugate_gradient(op, std::make_shared<ScaleOp>(ScaleOp{0.5}), 10, 1, 0.0)(x10, b);
+        report("PCG (P = I/2), 10 steps vs plain CG", rel_l2(x10.to_host(), xc.to_host()), 1e-5);
+        pcg(xp, b, std::function<bool(tensor<cuda_t, c64_t, 3>&)>([&](tensor<cuda_t, c64_t, 3>&) { return ++steps == 4; }),
+            std::function<void(tensor<cuda_t, c64_t, 3>&)>([&](tensor<cuda_t, c64_t, 3>&) { ++outers; }));
+        const auto Axp = (*op)(xp).to_host();
+        report("PCG with a restart: ||A x - b|| / ||b||", rel_l2(Axp, hb), 1e-3);
+        report("PCG callbacks: 24 steps, 3 outer iterations (0 = yes)", steps == 24 && outers == 3 ? 0.0 : 1.0, 0.5);
         auto [lam, v] = max_eig(*op, dx.clone(), 20);
         std::printf("[cpp] max_eig estimate %.6f\n", double(lam));
         if (!(lam > 0.f) || !std::isfinite(lam)) ++failures;

@@ -81,3 +81,77 @@ def test_max_eig_matches_oracle():
     lam, x = max_eig(lambda v: dg * v, cuda(x0), 30)
     lam_ref, _ = max_eig_ref(lambda v: d * v, x0, 30)
     assert abs(lam - lam_ref) <= 1e-5 * lam_ref, (lam, lam_ref)
+
+
+def _spd(rng, n, cond=30.0):
+    Q = np.linalg.qr(crandn(rng, (n, n)))[0]
+    A = Q @ np.diag(np.linspace(1, 6, n)) @ Q.conj().T + np.diag(np.linspace(1, cond, n))
+    return A.astype(np.complex64).astype(np.complex128)     # the matrix the GPU applies
+
+
+def test_pcg_jacobi_matches_oracle_and_dense_solve():
+    """Preconditioned CG (``cg.cppm:16-119``) with a Jacobi preconditioner on a
+    dense SPD matrix applied in complex64: early iterates against the oracle's
+    FP64 PCG, the solution after n steps against numpy.linalg.solve."""
+    from hastycucompute_amd.op import conjugate_gradient
+    from oracle.solvers import preconditioned_conjugate_gradient as pcg_ref
+    rng = np.random.default_rng(30)
+    n = 48
+    A = _spd(rng, n)
+    J = (1.0 / np.diag(A).real).astype(np.float32)
+    gA, gJ = cuda(A.astype(np.complex64)), cuda(J.astype(np.complex64))
+    Af = lambda v: torch.matmul(gA, v)
+    Pf = lambda r: gJ * r
+    b = crandn(rng, n)
+    for k in (1, 3, 6):
+        x = torch.zeros(n, dtype=torch.complex64, device="cuda")
+        conjugate_gradient(Af, Pf, k, 1, 0.0)(x, cuda(b))
+        ref = pcg_ref(lambda v: A @ v, lambda r: J * r, b, np.zeros(n), k, 1, 0.0)
+        report(f"PCG (Jacobi) {k} steps vs oracle FP64 PCG", rel_l2(host(x), ref), 2e-5)
+    x = torch.zeros(n, dtype=torch.complex64, device="cuda")
+    conjugate_gradient(Af, Pf, 2 * n, 1, 0.0)(x, cuda(b))
+    report("PCG (Jacobi) 2n steps vs dense FP64 solve", rel_l2(host(x), np.linalg.solve(A, b)), 1e-4)
+
+
+def test_pcg_inverse_preconditioner_converges_in_one_step():
+    """With P = A^-1 the first direction is A^-1 r0 and alpha = 1, so one step
+    reaches the solution; the device flag then stops the rest."""
+    from hastycucompute_amd.op import conjugate_gradient
+    rng = np.random.default_rng(31)
+    n = 40
+    A = _spd(rng, n, cond=5.0)
+    gA = cuda(A.astype(np.complex64))
+    gAi = cuda(np.linalg.inv(A).astype(np.complex64))
+    b = crandn(rng, n)
+    x = torch.zeros(n, dtype=torch.complex64, device="cuda")
+    conjugate_gradient(lambda v: torch.matmul(gA, v), lambda r: torch.matmul(gAi, r), 1, 1, 0.0)(x, cuda(b))
+    report("PCG with P = A^-1, one step vs dense solve", rel_l2(host(x), np.linalg.solve(A, b)), 2e-5)
+
+
+def test_pcg_restart_callbacks_match_oracle():
+    """should_restart_callback(x) true on the 3rd and 7th step resets p = z and
+    ends that outer iteration; after_restart_callback runs once per outer
+    iteration; the iterate matches the oracle run with the same schedule."""
+    from hastycucompute_amd.op import conjugate_gradient
+    from oracle.solvers import preconditioned_conjugate_gradient as pcg_ref
+    rng = np.random.default_rng(32)
+    n = 64
+    A = _spd(rng, n)
+    J = (1.0 / np.diag(A).real)
+    gA, gJ = cuda(A.astype(np.complex64)), cuda(J.astype(np.complex64))
+    b = crandn(rng, n)
+
+    def schedule():
+        step = [0]
+
+        def should_restart(_x):
+            step[0] += 1
+            return step[0] in (3, 7)
+        return should_restart
+    after = []
+    x = torch.zeros(n, dtype=torch.complex64, device="cuda")
+    conjugate_gradient(lambda v: torch.matmul(gA, v), lambda r: gJ * r, 5, 3, 0.0)(
+        x, cuda(b), schedule(), lambda _x: after.append(1))
+    ref = pcg_ref(lambda v: A @ v, lambda r: J * r, b, np.zeros(n), 5, 3, 0.0, should_restart=schedule())
+    assert len(after) == 3
+    report("PCG with restarts at steps 3 and 7 vs oracle", rel_l2(host(x), ref), 5e-5)

@@ -41,6 +41,31 @@ def test_toeplitz_kernel_weighted():
     report("toeplitz_kernel weighted", rel_l2(host(k), ot.toeplitz_kernel(coords, shape, w)), 2e-6)
 
 
+@pytest.mark.parametrize("shape,C,M", [((10, 12, 8), 3, 3000), ((24, 20), 2, 2500)])
+def test_reference_sequence_kernel_transform_multiplication(shape, C, M):
+    """The reference's call sequence (``toeplitz.cppm:510-712``): toeplitz_kernel,
+    then transform_toeplitz_kernel, then toeplitz_multiplication with the SENSE
+    modes of NORMAL_T_T1_OP (input S_c, mult1 = x, in/out MULT_CONJ, in_m1 MULT),
+    then the coil sum, against the FP64 NDFT normal operator.  The transform
+    leaves this library's kernel as it is, bit for bit."""
+    from hastycucompute_amd import fft as hf
+    rng = np.random.default_rng(17)
+    coords = rand_coords(rng, len(shape), M)
+    S = crandn(rng, (C,) + shape)
+    x = crandn(rng, shape)
+    k = _kernel_gpu(coords, shape)
+    before = k.clone()
+    hf.transform_toeplitz_kernel(k, False)
+    assert torch.equal(k, before)
+    out = torch.empty((C,) + shape, dtype=torch.complex64, device="cuda")
+    T = hf.ToeplitzMultType
+    hf.toeplitz_multiplication(cuda(S), out, k, None, cuda(x), None, T.MULT_CONJ, T.MULT, T.NONE, T.NONE,
+                               T.NONE, hf.ToeplitzAccumulateType.NONE)
+    got = host(out).astype(np.complex128).sum(0)
+    report(f"toeplitz_kernel -> transform -> multiplication + sum {shape} C={C}",
+           rel_l2(got, sense_normal_direct(x, S, coords)), TOL)
+
+
 @pytest.mark.parametrize("shape", [(8, 10, 6), (16, 12, 20), (32, 24), (5, 6, 3)])
 def test_toeplitz_multiply(shape):
     from hastycucompute_amd import fft as hf

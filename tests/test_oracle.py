@@ -226,6 +226,41 @@ def test_cg_exact_and_dense():
     assert hist[-1] < hist[0]
 
 
+def test_preconditioned_cg_pins():
+    """preconditioned_conjugate_gradient (``cg.cppm:16-119``): P = I gives the
+    plain CG iterates; P = A^-1 lands on the solution in one step (p0 = A^-1 r0,
+    alpha = 1); a Jacobi P converges to numpy.linalg.solve in n steps of one
+    outer iteration, and in 2 outer iterations of n/2 steps when nothing
+    restarts; a restart on every step is preconditioned steepest descent
+    (x += (r.z / z.Az) z), written out here."""
+    from oracle.solvers import preconditioned_conjugate_gradient as pcg
+    rng = np.random.default_rng(11)
+    n = 16
+    Q = np.linalg.qr(crandn(rng, (n, n)))[0]
+    A = Q @ np.diag(np.linspace(1, 8, n)) @ Q.conj().T + np.diag(np.linspace(1, 30, n))
+    Af = lambda v: A @ v
+    b = crandn(rng, n)
+    x0 = np.zeros(n, complex)
+    for k in (1, 4, 9):
+        assert rel_l2(pcg(Af, lambda r: r.copy(), b, x0, k, 1, 0.0), conjugate_gradient(Af, b, x0, k)) < 1e-12
+    Ainv = np.linalg.inv(A)
+    assert rel_l2(pcg(Af, lambda r: Ainv @ r, b, x0, 1, 1, 0.0), np.linalg.solve(A, b)) < 1e-12
+    J = 1.0 / np.diag(A).real
+    xs = np.linalg.solve(A, b)
+    assert rel_l2(pcg(Af, lambda r: J * r, b, x0, n, 1, 0.0), xs) < 1e-9
+    assert rel_l2(pcg(Af, lambda r: J * r, b, x0, n // 2, 2, 0.0), xs) < 1e-9
+    calls = []
+    got = pcg(Af, lambda r: J * r, b, x0, 3, 4, 0.0, should_restart=lambda x: True,
+              after_restart=lambda x: calls.append(1))
+    assert len(calls) == 4
+    x = x0.copy()
+    for _ in range(4):          # one step per outer iteration
+        r = b - A @ x
+        z = J * r
+        x = x + (np.vdot(r, z).real / np.vdot(z, A @ z).real) * z
+    assert rel_l2(got, x) < 1e-12
+
+
 def test_max_eig():
     rng = np.random.default_rng(8)
     n = 10
