@@ -257,6 +257,7 @@ _SQ_PASSES = ("SQ_INSTS_VALU SQ_INSTS_SALU SQ_INSTS_SMEM SQ_INSTS_VMEM_RD SQ_WAI
               "SQ_BUSY_CYCLES SQ_ACTIVE_INST_VALU",
               "SQ_WAVES SQ_INSTS_VALU_FMA_F32 SQ_INSTS_LDS SQ_LDS_BANK_CONFLICT SQ_ACTIVE_INST_LDS GRBM_GUI_ACTIVE")
 _SQ_KERNELS = (("spread_kernel_w6<c64, 32>", "spread_kernel_w6<HIP_vector_type<float, 2u>, 32>", 20_000_000),
+               ("spread_lanes32_kernel<6>", "spread_lanes32_kernel<6>", 10_000_000),
                ("interp_pair_kernel<6, 16, 16, 16>", "interp_pair_kernel<6, 16, 16, 16>", 40_000_000))
 
 

@@ -287,15 +287,15 @@ __global__ void fold_sorted_kernel(FoldArgs<CT> a, const int* sidx, int* sinv, i
               (1 + second_tile(a.g, 2, l0w[2]));
 }
 
-// The (tile, 8 s + neighbour bits) pair of every visit, in the caller's point
-// order (vpos = exclusive scan of vcnt over j); a stable sort by tile then
-// groups them, keeping that order inside each tile (consecutive visits of a
-// tile then spread along the acquisition, e.g. along spokes, which balances
-// the sub-blocks of a tile within each chunk of visits).
+// The (tile, 8 s + neighbour bits) pair of every visit, in emission order:
+// thread i emits point j = (i * stride) mod M at vpos[i] (vpos = exclusive
+// scan of the counts in that order, permute_counts_kernel); a stable sort by
+// tile then groups them, keeping that order inside each tile.
 __global__ void visit_emit_kernel(const int* sl0, const int* sinv, int64_t M, Geo g, const int* vpos,
-                                  unsigned* vkey, int* vval) {
-    const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (j >= M) return;
+                                  unsigned* vkey, int* vval, int64_t stride) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    const int64_t j = (i * stride) % M;   // emission order (see visit_stride)
     const int64_t s = sinv[j];
     int l0w[3], t3[3];
     for (int d = 0; d < 3; ++d) {
@@ -303,7 +303,7 @@ __global__ void visit_emit_kernel(const int* sl0, const int* sinv, int64_t M, Ge
         t3[d] = l0w[d] / g.T[d];
     }
     const int e0 = second_tile(g, 0, l0w[0]), e1 = second_tile(g, 1, l0w[1]), e2 = second_tile(g, 2, l0w[2]);
-    int v = vpos[j];
+    int v = vpos[i];
     for (int bz = 0; bz <= e0; ++bz)
         for (int by = 0; by <= e1; ++by)
             for (int bx = 0; bx <= e2; ++bx) {
@@ -312,6 +312,25 @@ __global__ void visit_emit_kernel(const int* sl0, const int* sinv, int64_t M, Ge
                 vval[v] = int(s) * 8 + (bz << 2 | by << 1 | bx);
                 ++v;
             }
+}
+
+// counts in emission order: out[i] = vcnt[(i * stride) mod M]
+__global__ void permute_counts_kernel(const int* vcnt, int* out, int64_t M, int64_t stride) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < M) out[i] = vcnt[(i * stride) % M];
+}
+
+// Emission order of the visits: point j = (i * stride) mod M is emitted i-th,
+// with stride the integer nearest 0.618 M that is coprime to M (a golden-ratio
+// low-discrepancy permutation).  After the stable sort by tile, the visit list
+// of a tile then interleaves points from all over the acquisition instead of
+// runs of neighbouring samples along one spoke, so each chunk of visits is
+// spread evenly over the tile's sub-blocks.  The order is fixed for a given M.
+static int64_t visit_stride(int64_t M) {
+    if (M <= 1) return 1;
+    int64_t P = std::max<int64_t>(1, int64_t(0.6180339887498949 * double(M)));
+    while (std::gcd(P, M) != 1) ++P;
+    return P;
 }
 
 __global__ void visit_records_kernel(const int* vval, const int* sl0, int64_t M, int64_t nvis, int4* visits) {
@@ -780,12 +799,17 @@ void nufft_setpts_typed(NufftPlan* p, int64_t M, const void* c0, const void* c1,
         else if (p->f64) fold_sorted((float*)nullptr, (double*)nullptr);
         else fold_sorted((float*)nullptr, (float*)nullptr);
         HASTY_CHECK_HIP(hipGetLastError());
-        // 3. visits in sorted point order, then a stable sort by tile
+        // 3. visits in emission order (visit_stride), then a stable sort by tile
+        const int64_t stride = visit_stride(M);
+        int* vcnt_e = dev_alloc<int>(M, tally_tmp);
+        hipLaunchKernelGGL(permute_counts_kernel, dim3(blocks), dim3(256), 0, s, static_cast<const int*>(vcnt),
+                           vcnt_e, M, stride);
+        HASTY_CHECK_HIP(hipGetLastError());
         {
             size_t tb = 0;
-            HASTY_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, vcnt, vpos, M, s));
+            HASTY_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, vcnt_e, vpos, M, s));
             void* tmp = dev_alloc<unsigned char>(int64_t(std::max<size_t>(tb, 1)), tally_tmp);
-            HASTY_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tb, vcnt, vpos, M, s));
+            HASTY_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tb, vcnt_e, vpos, M, s));
             HASTY_CHECK_HIP(hipStreamSynchronize(s));
             HASTY_CHECK_HIP(hipFree(tmp));
         }
@@ -795,7 +819,7 @@ void nufft_setpts_typed(NufftPlan* p, int64_t M, const void* c0, const void* c1,
         int* vval_out = dev_alloc<int>(voff, tally_tmp);
         hipLaunchKernelGGL(visit_emit_kernel, dim3(blocks), dim3(256), 0, s, static_cast<const int*>(p->sl0),
                            static_cast<const int*>(p->sinv), M, g, static_cast<const int*>(vpos), vkey_in,
-                           vval_in);
+                           vval_in, stride);
         HASTY_CHECK_HIP(hipGetLastError());
         sort_pairs(vkey_in, vkey_out, vval_in, vval_out, voff, key_bits(nt));
         hipLaunchKernelGGL(visit_records_kernel, dim3(unsigned(ceil_div(voff, 256))), dim3(256), 0, s,
@@ -804,7 +828,8 @@ void nufft_setpts_typed(NufftPlan* p, int64_t M, const void* c0, const void* c1,
         HASTY_CHECK_HIP(hipGetLastError());
         HASTY_CHECK_HIP(hipStreamSynchronize(s));
         for (void* q : {static_cast<void*>(keys_in), static_cast<void*>(keys_out), static_cast<void*>(idx_in),
-                        static_cast<void*>(vcnt), static_cast<void*>(vpos), static_cast<void*>(vkey_in),
+                        static_cast<void*>(vcnt), static_cast<void*>(vcnt_e), static_cast<void*>(vpos),
+                        static_cast<void*>(vkey_in),
                         static_cast<void*>(vkey_out), static_cast<void*>(vval_in), static_cast<void*>(vval_out)})
             HASTY_CHECK_HIP(hipFree(q));
     }
@@ -832,6 +857,21 @@ using SpreadFn = void (*)(GridArgs<V>, const int4*, const V*, const decltype(V::
 // 32 tiles (a row of tiles along x shares its straddling points in one L2:
 // 28.0 -> 24.7 ms); interpolation is fastest with single subproblems
 // interleaved over the XCDs.  HASTY_XCD_CHUNK overrides both (tuning).
+// The lanes-as-transforms spreading kernel, when it applies: complex64 3-D
+// plans spreading 32 transforms per block, an even batch (16-B aligned value
+// rows), every fine-grid axis >= w + 4 (a support meets a 4-cell sub-block in
+// one piece).  HASTY_SPREAD_LANES=0 keeps the cell-per-lane kernel.
+static void* spread_lanes_fn(const NufftPlan* p, int nb) {
+    static const bool off = [] {
+        const char* e = std::getenv("HASTY_SPREAD_LANES");
+        return e && e[0] == '0';
+    }();
+    if (off || p->f64 || p->dim != 3 || p->group != 32 || nb % 2 != 0) return nullptr;
+    for (int a = 0; a < 3; ++a)
+        if (p->nf[a] < p->w + 4) return nullptr;
+    return nufft_spread_lanes32_ptr(p->w);
+}
+
 static int xcd_chunk(bool spread) {
     static const int env = [] {
         const char* e = std::getenv("HASTY_XCD_CHUNK");
@@ -950,7 +990,18 @@ void execute_impl(NufftPlan* p, V* c, V* f, const NufftHooks& hk, int ntransf) {
                     HASTY_CHECK_HIP(hipGetLastError());
                 }
                 ga.subs = p->sub_spread;
-                {
+                if (void* lanes_fn = spread_lanes_fn(p, nb)) {
+                    HASTY_PROF("nufft_spread", s);
+                    const int ng = int(ceil_div(nb, 32));
+                    ga.nsub = p->nsub_spread;
+                    ga.ngroups = ng;
+                    ga.xchunk = xcd_chunk(true);
+                    hipLaunchKernelGGL(reinterpret_cast<SpreadFn<V>>(lanes_fn),
+                                       dim3(unsigned(xcd_grid(p->nsub_spread, ng, ga.xchunk))),
+                                       dim3(kLaneSpreadThreads), 0, s, ga, static_cast<const int4*>(p->visits),
+                                       static_cast<const V*>(p->csort), static_cast<const R*>(p->sw));
+                    HASTY_CHECK_HIP(hipGetLastError());
+                } else {
                     HASTY_PROF("nufft_spread", s);
                     // one wave per 64-cell sub-block of the tile, up to 8 per block
                     const int b0 = p->T[0] == 1 ? 1 : 4, b1 = p->T[0] == 1 ? 8 : 4, b2 = b1;

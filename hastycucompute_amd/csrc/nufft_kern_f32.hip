@@ -30,6 +30,20 @@ template <> void* nufft_spread_ptr<float2>(int cg) {
     throw Error(HASTY_ERR_INTERNAL, "no spreading kernel for this transform group");
 }
 
+void* nufft_spread_lanes32_ptr(int w) {
+    switch (w) {
+        case 2: return reinterpret_cast<void*>(&spread_lanes32_kernel<2>);
+        case 3: return reinterpret_cast<void*>(&spread_lanes32_kernel<3>);
+        case 4: return reinterpret_cast<void*>(&spread_lanes32_kernel<4>);
+        case 5: return reinterpret_cast<void*>(&spread_lanes32_kernel<5>);
+        case 6: return reinterpret_cast<void*>(&spread_lanes32_kernel<6>);
+        case 7: return reinterpret_cast<void*>(&spread_lanes32_kernel<7>);
+        case 8: return reinterpret_cast<void*>(&spread_lanes32_kernel<8>);
+        case 9: return reinterpret_cast<void*>(&spread_lanes32_kernel<9>);
+    }
+    return nullptr;
+}
+
 void* nufft_interp_pair_ptr(int dim, int w) {
     // 3-D only: a 2-D window is one z plane, whose 16-B reads the compiler
     // hoists together past the register budget

@@ -445,6 +445,202 @@ __global__ void __launch_bounds__(kNufftThreads) __attribute__((amdgpu_waves_per
     spread_body<V, CG>(a, visits, csort, sw);
 }
 
+// ---------------------------------------------------------------------------
+// Spreading with the transforms on the lanes (complex64, 32 transforms per
+// block, 3-D).  Lane l of a wave holds float l of a point's 32 complex values
+// (transform l / 2, real or imaginary part), so the whole wave works on one
+// point at a time and every weight is wave-uniform: a (z, y) row of the
+// wave's 4 x 4 x 4 sub-block that the point's support misses is skipped by a
+// scalar branch, so only in-support rows do work (1 + 4 FMA-class ops; x
+// cells outside the support take zero weights).  Each lane keeps 64 FP32
+// accumulators: one cell of the sub-block each, for its transform and part.
+//
+// The block (8 waves, one 4^3 sub-block each; larger tiles take several
+// passes) walks the tile's visit list in chunks of 64 records.  Every chunk
+// is staged once in LDS for all 8 waves: the records, the points' 32 complex
+// values (256 B each) and their window weights as zero-padded rows (index
+// q + 3 for q = cell - first support cell, so any cell of a sub-block reads
+// a weight or a zero).  Each wave tests the 64 records against its sub-block
+// and takes the hits in visit order, so every cell's sum has a fixed order
+// and the result is the same on every run.  No atomics.  Requires every axis
+// of the fine grid to be >= w + 4 (a support then meets a sub-block in one
+// piece).
+// ---------------------------------------------------------------------------
+constexpr int kLaneSpreadThreads = 512;
+constexpr int kLaneSpreadChunk = 128;   // records staged per chunk (a multiple of 64)
+constexpr int kLaneSpreadPad = 16;   // padded weight row (w + 6 <= 16)
+
+__device__ __forceinline__ int lane_hit_pop(unsigned long long& mask) {
+    const int i = __ffsll(mask) - 1;
+    mask &= mask - 1;
+    return i;
+}
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+template <int W>
+__global__ void __launch_bounds__(kLaneSpreadThreads)
+    spread_lanes32_kernel(GridArgs<float2> a, const int4* __restrict__ visits,
+                          const float2* __restrict__ csort, const float* __restrict__ sw) {
+    static_assert(W + 6 <= kLaneSpreadPad, "padded weight row too short");
+    static_assert(kLaneSpreadChunk % 64 == 0 && kLaneSpreadThreads == 512, "staging: 8 threads per record");
+    constexpr int K = kLaneSpreadChunk, PW = kLaneSpreadPad;
+    __shared__ int4 lrec[K];
+    __shared__ float lval[K * 64];
+    __shared__ float lw[K * 3 * PW];
+    int isub, igroup;
+    if (!xcd_block(a.nsub, a.ngroups, a.xchunk, isub, igroup)) return;
+    const int4 sub = a.subs[isub];
+    int t0[3];
+    tile_origin(a.g, sub.x, t0);
+    const int T0 = a.g.T[0], T1 = a.g.T[1], T2 = a.g.T[2];
+    const int cells = T0 * T1 * T2;
+    const int n0 = int(a.g.nf[0]), n1 = int(a.g.nf[1]), n2 = int(a.g.nf[2]);
+    const int ns1 = (T1 + 3) / 4, ns2 = (T2 + 3) / 4;
+    const int nsb = ((T0 + 3) / 4) * ns1 * ns2;
+    const int g0 = igroup * 32;
+    const int ng = min(32, a.nbatch - g0);
+    const int lane = int(threadIdx.x) & 63;
+    const int wave = wave_uniform(int(threadIdx.x >> 6));
+    const int64_t G = a.g.nf[0] * a.g.nf[1] * a.g.nf[2];
+    const int vend = sub.y + sub.z;
+    const int npass = (nsb + 7) / 8;
+    // padded weight rows: entries q + 3 for taps q < W are written per chunk,
+    // the rest stay zero (the barrier orders these zeros before the first
+    // chunk's weights, which land on some of the same entries)
+    for (int e = int(threadIdx.x); e < K * 3 * PW; e += kLaneSpreadThreads) lw[e] = 0.f;
+    __syncthreads();
+    for (int pass = 0; pass < npass; ++pass) {
+        const int sb = pass * 8 + wave;
+        const bool active = sb < nsb;
+        const int o2 = (sb % ns2) * 4, o1 = ((sb / ns2) % ns1) * 4, o0 = (sb / (ns2 * ns1)) * 4;
+        const int G0 = t0[0] + o0, G1 = t0[1] + o1, G2 = t0[2] + o2;   // sub-block origin
+        // acc[m][n]: the 32 x 32 MFMA tile of transform-parts 32 m .. 32 m + 31
+        // by sub-block cells 32 n .. 32 n + 31 (cell = 16 z + 4 y + x)
+        floatx16 acc[2][2];
+#pragma unroll
+        for (int m = 0; m < 2; ++m)
+#pragma unroll
+            for (int n = 0; n < 2; ++n)
+#pragma unroll
+                for (int v = 0; v < 16; ++v) acc[m][n][v] = 0.f;
+        // the next chunk's records, loaded while the current chunk is added up
+        int4 nrec_pf[K / 64];
+#pragma unroll
+        for (int rb = 0; rb < K; rb += 64) {
+            const int v = sub.y + rb + (int(threadIdx.x) >> 3);
+            nrec_pf[rb / 64] = v < vend ? visits[v] : make_int4(0, 0, 0, 0);
+        }
+        for (int vb = sub.y; vb < vend; vb += K) {
+            // ---- stage the chunk: records, values, weights ----
+#pragma unroll
+            for (int rb = 0; rb < K; rb += 64) {
+                const int t = int(threadIdx.x);
+                const int r = rb + (t >> 3), j = t & 7;
+                if (vb + r < vend) {
+                    const int4 rec = nrec_pf[rb / 64];
+                    const int s = rec.x >> 3;
+                    if (j == 0) lrec[r] = rec;
+                    const float4* src = reinterpret_cast<const float4*>(csort + int64_t(s) * a.nbatch + g0) + 2 * j;
+                    const float4 u0 = src[0], u1 = src[1];
+                    // the 3W weights (rows padded to 16 B in sw): float4 j < ceil(3W/4)
+                    float4 wq = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (4 * j < 3 * W) wq = reinterpret_cast<const float4*>(sw + int64_t(s) * a.ws)[j];
+                    float4* dst = reinterpret_cast<float4*>(lval + r * 64) + 2 * j;
+                    dst[0] = u0;
+                    dst[1] = u1;
+                    const float wv[4] = {wq.x, wq.y, wq.z, wq.w};
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const int k = 4 * j + e;     // weight k = axis k / W, tap k % W
+                        if (k < 3 * W) lw[r * 3 * PW + (k / W) * PW + (k % W) + 3] = wv[e];
+                    }
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int rb = 0; rb < K; rb += 64) {
+                const int v = vb + K + rb + (int(threadIdx.x) >> 3);
+                if (v < vend) nrec_pf[rb / 64] = visits[v];
+            }
+            const int nrec = min(K, vend - vb);
+            for (int rb = 0; active && rb < nrec; rb += 64) {
+                int4 rec = make_int4(0, 0, 0, 0);
+                bool hit = false;
+                if (rb + lane < nrec) {
+                    rec = lrec[rb + lane];
+                    hit = arcs_meet(rec.y, W, G0, 4, n0) && arcs_meet(rec.z, W, G1, 4, n1) &&
+                          arcs_meet(rec.w, W, G2, 4, n2);
+                }
+                // each lane prepares its own record: the offsets of the
+                // point's first support cell from the sub-block origin, in
+                // [-(W - 1), 3] (periodic), as padded-row starts 3 - o, and
+                // which z planes / y rows of the sub-block the support covers
+                int packed = 0;
+                if (hit) {
+                    int oz = rec.y - G0, oy = rec.z - G1, ox = rec.w - G2;
+                    oz += oz < -(W - 1) ? n0 : 0; oz -= oz > 3 ? n0 : 0;
+                    oy += oy < -(W - 1) ? n1 : 0; oy -= oy > 3 ? n1 : 0;
+                    ox += ox < -(W - 1) ? n2 : 0; ox -= ox > 3 ? n2 : 0;
+                    packed = (3 - oz) | ((3 - oy) << 4) | ((3 - ox) << 8);
+                }
+                // hits in visit order, two per MFMA step: lanes 0-31 carry hit
+                // k = 0 and lanes 32-63 hit k = 1 of the pair (an odd last hit
+                // pairs with zero weights)
+                const int k = lane >> 5, j = lane & 31;
+                const int cz = j >> 4, cy = (j >> 2) & 3, cx = j & 3;   // cell j of N-tile 0 (N-tile 1: z + 2)
+                unsigned long long mask = __ballot(hit);
+                while (mask) {
+                    const int i0 = lane_hit_pop(mask);
+                    const int i1 = mask ? lane_hit_pop(mask) : -1;
+                    const int ih = (k == 0 || i1 < 0) ? i0 : i1;
+                    const int pk = __builtin_amdgcn_readlane(packed, i0) * (1 - k) +
+                                   (i1 >= 0 ? __builtin_amdgcn_readlane(packed, i1) : 0) * k;
+                    const float* vr = lval + (rb + ih) * 64;
+                    const float a0 = vr[j], a1 = vr[32 + j];
+                    const float* wr = lw + (rb + ih) * 3 * PW;
+                    const float wzr0 = wr[(pk & 15) + cz], wzr1 = wr[(pk & 15) + cz + 2];
+                    const float wyx = wr[PW + ((pk >> 4) & 15) + cy] * wr[2 * PW + ((pk >> 8) & 15) + cx];
+                    const bool live = k == 0 || i1 >= 0;
+                    const float b0 = live ? wzr0 * wyx : 0.f, b1 = live ? wzr1 * wyx : 0.f;
+                    acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, acc[0][0], 0, 0, 0);
+                    acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b1, acc[0][1], 0, 0, 0);
+                    acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b0, acc[1][0], 0, 0, 0);
+                    acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, acc[1][1], 0, 0, 0);
+                }
+            }
+            __syncthreads();
+        }
+        if (!active) continue;
+        // ---- write the sub-block's cells of this tile ----
+        // 32 x 32 MFMA output: register v of lane l holds row
+        // 8 (v / 4) + 4 (l / 32) + v % 4 and column l % 32 of its tile.
+        // (An opaque copy of the lane index keeps the compiler from hoisting
+        // these 64 addresses out of the loops into registers.)
+        int lw_ = lane;
+        asm volatile("" : "+v"(lw_));
+#pragma unroll
+        for (int m = 0; m < 2; ++m)
+#pragma unroll
+            for (int n = 0; n < 2; ++n)
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    const int tp = 32 * m + 8 * (v >> 2) + 4 * (lw_ >> 5) + (v & 3);   // transform-part
+                    const int cell_sb = 32 * n + (lw_ & 31);
+                    const int lz = o0 + (cell_sb >> 4), ly = o1 + ((cell_sb >> 2) & 3), lx = o2 + (cell_sb & 3);
+                    if ((tp >> 1) >= ng || lz >= T0 || ly >= T1 || lx >= T2) continue;
+                    const int cell = (lz * T1 + ly) * T2 + lx;
+                    float* dst;
+                    if (sub.w != 0)
+                        dst = reinterpret_cast<float*>(a.partial + (int64_t(sub.w - 1) * a.nbatch + g0 + (tp >> 1)) * cells + cell);
+                    else
+                        dst = reinterpret_cast<float*>(a.fine + int64_t(g0 + (tp >> 1)) * G +
+                                                       (int64_t(t0[0] + lz) * n1 + (t0[1] + ly)) * n2 + (t0[2] + lx));
+                    dst[tp & 1] = acc[m][n][v];
+                }
+    }
+}
+
 // Sum the partial tiles of every split tile into the fine grid.
 // info[k] = (tile, first slot, slot count, -).  Deterministic order.
 template <class V>
@@ -733,5 +929,8 @@ template <class V> void* nufft_interp_fixed_ptr(int dim, int w);
 void* nufft_interp_pair_ptr(int dim, int w);
 // spreading kernel for CG transforms per block (1, 2, 4, 8, 16, 32 / 16 for complex128)
 template <class V> void* nufft_spread_ptr(int cg);
+// complex64 spreading with the 32 transforms of a block on the lanes
+// (spread_lanes32_kernel), or nullptr for widths without an instance
+void* nufft_spread_lanes32_ptr(int w);
 
 }  // namespace hasty
