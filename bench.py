@@ -90,6 +90,69 @@ def fp64_ndft_type2_subset(img, coords, idx):
     return torch.cat(out)
 
 
+def fp64_direct_type1_subset(c, coords, shape, vox):
+    """f[n] = sum_j c_j exp(+i k_j . n) in FP64 on the GPU for the centred mode
+    indices n = vox [V, 3] (all M points, in chunks of 2^20)."""
+    import torch
+    n = vox.to(torch.float64)
+    out = torch.zeros(vox.shape[0], dtype=torch.complex128, device=c.device)
+    M = c.shape[0]
+    for j0 in range(0, M, 1 << 20):
+        j1 = min(M, j0 + (1 << 20))
+        ph = sum(coords[a][j0:j1].to(torch.float64)[:, None] * n[None, :, a] for a in range(3))
+        out += c[j0:j1].to(torch.complex128) @ torch.polar(torch.ones_like(ph), ph)
+    return out
+
+
+def fp64_plan_pair_normal(coords, smaps, x, N, tol=1e-9):
+    """(1/N^3) sum_c conj(S_c) A1(A2(S_c x)) with complex128 NUFFT plans at tol."""
+    import torch
+    from hastycucompute_amd import fft as hf
+    C = smaps.shape[0]
+    c64 = [c.to(torch.float64) for c in coords]
+    img = (smaps.to(torch.complex128) * x.to(torch.complex128)[None]).contiguous()
+    opts = dict(nmodes=(N, N, N), ntransf=C, tol=tol)
+    q2 = hf.nufft_plan.make(hf.nufft_opts(sign=-1, **opts), hf.nufft_type.TYPE_2, torch.float64)
+    q2.setpts(c64)
+    y = torch.empty((C, c64[0].shape[0]), dtype=torch.complex128, device=x.device)
+    q2.execute(img, y)
+    q2.free()
+    q1 = hf.nufft_plan.make(hf.nufft_opts(sign=+1, **opts), hf.nufft_type.TYPE_1, torch.float64)
+    q1.setpts(c64)
+    q1.execute(y, img)
+    q1.free()
+    del y
+    return (torch.conj(smaps.to(torch.complex128)) * img).sum(0) / float(N ** 3)
+
+
+def cpu_baseline_nufft(N, C, coords_np, smaps_np, x_np, sample_coils):
+    """Time the oracle's CPU restatement of the NUFFT (the KB gridding step in
+    C with OpenMP, scipy.fft) on every host core, for a sample of the coils
+    over the whole trajectory; Mpts/s extrapolated to all coils."""
+    import numpy as np
+
+    from oracle.kb import KBPlan
+    cores = len(os.sched_getaffinity(0))
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    plan = KBPlan((N, N, N), 1e-5, dtype=np.complex64, workers=cores).setpts(coords_np)
+    f = (smaps_np[:sample_coils] * x_np[None]).astype(np.complex64)
+    t0 = time.perf_counter()
+    yk = plan.type2(f)
+    t2 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    plan.type1(yk)
+    t1 = time.perf_counter() - t0
+    M = len(coords_np[0])
+    scale = C / sample_coils
+    return {"type2_mpts_s": round(C * M / (t2 * scale) / 1e6, 2),
+            "type1_mpts_s": round(C * M / (t1 * scale) / 1e6, 2),
+            "unit": "Mpts/s", "cores": cores, "kind": "port",
+            "sample": f"oracle KB NUFFT (complex64, C gridding with OpenMP + scipy.fft, {cores} threads), "
+                      f"{sample_coils} of {C} coils over all {M} points at {N}^3, one run each, "
+                      f"extrapolated x{scale:g}",
+            "seconds": {"type2": round(t2, 3), "type1": round(t1, 3)}}
+
+
 def cpu_baseline(N, C, smaps_np, x_np, kernel_np, sample_coils):
     """Time the oracle's CPU restatement of the Toeplitz normal operator on a
     bounded sample of coils (complex64, scipy.fft on every host core)."""
@@ -403,6 +466,7 @@ def main():
     ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-coils", type=int, default=2)
+    ap.add_argument("--cpu-nufft-coils", type=int, default=4)
     ap.add_argument("--phases", default="aha,type2,type1,plain,segments,frames")
     ap.add_argument("--reduce-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="reduce two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of this script "
@@ -500,6 +564,31 @@ def main():
     if "type1" in phases:
         res["type1"] = run_phase("type1", lambda: p1.execute(y, back))
 
+    # 30 CG iterations on the normal equations N x = b, b = N x_true (FP64
+    # dot products and scalars on the device), timed
+    cg30 = None
+    if "aha" in phases:
+        from hastycucompute_amd.op import conjugate_gradient
+        b = torch.empty_like(x)
+        op.apply(x, b)
+        xs = torch.zeros_like(x)
+        hist = []
+        conjugate_gradient(op, 30, 0.0)(xs, b)          # warm
+        xs.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        conjugate_gradient(op, 30, 0.0)(xs, b, history=hist)
+        torch.cuda.synchronize()
+        cg_s = time.perf_counter() - t0
+        r_true = torch.empty_like(x)
+        op.apply(xs, r_true)
+        bn = float(b.to(torch.complex128).norm())
+        cg30 = {"iters": 30, "ms": round(cg_s * 1e3, 2), "ms_per_iter": round(cg_s * 1e3 / 30, 3),
+                "recurrence_residual_rel": float(hist[-1].item()) / bn,
+                "true_residual_rel": float((b - r_true).to(torch.complex128).norm()) / bn,
+                "error_vs_truth_rel": float((xs - x).to(torch.complex128).norm() / x.to(torch.complex128).norm())}
+        log(f"[bench] CG 30 iterations: {cg30}")
+
     # the plain per-coil GPU path, as the reference runs it: one coil per
     # normal-operator launch sequence, one single-transform NUFFT plan looped
     # over the coils (nufft.cppm:316-382 style, without its host copies)
@@ -558,7 +647,25 @@ def main():
     pair = (torch.conj(smaps) * back).sum(0) / float(N ** 3)
     op.apply(x, out)
     acc["aha_vs_nufft_pair"] = float((out - pair).norm() / pair.norm())
+    del pair
+    # type 1 vs an FP64 direct sum on 256 random voxels, coils 0 and C - 1
+    vox = torch.stack([torch.randint(-(N // 2), (N + 1) // 2, (256,), generator=g) for _ in range(3)], 1).to(dev)
+    vidx = tuple((vox[:, a] + N // 2) for a in range(3))
+    errs = []
+    for cc in (0, C - 1):
+        ref1 = fp64_direct_type1_subset(y[cc], coords, (N, N, N), vox)
+        errs.append(float((back[cc][vidx].to(torch.complex128) - ref1).norm() / ref1.norm()))
+    acc["type1_vs_fp64_direct_256vox"] = max(errs)
+    # the whole A^H A vs complex128 NUFFT plans at tol 1e-9 (FP64 arithmetic)
+    if "aha" in phases:
+        ref_n = fp64_plan_pair_normal(coords, smaps, x, N)
+        op.apply(x, out)
+        acc["aha_vs_fp64_plans_tol1e-9"] = float((out.to(torch.complex128) - ref_n).norm() / ref_n.norm())
+        del ref_n
+        torch.cuda.empty_cache()
     log(f"[bench] accuracy: {acc}")
+
+
 
     ab = algorithmic_bytes(N, C, M)
     aha_t = res.get("aha")
@@ -575,6 +682,25 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": round(traffic) if traffic else None,
                 "algorithmic_bytes_per_launch": ab[dom], "avg_launch_ms": round(toep[dom][0], 4)}
+    # the NUFFT half of the metric: HBM view of interpolation and spreading,
+    # and the matrix-core view of spreading (8 C M w^3 useful flops: a
+    # complex accumulation per coil per window cell, run as FP32 MFMA)
+    roof_nufft = {}
+    w = p2.info()["width"]
+    for tag, kname in (("type2", "nufft_interp"), ("type1", "nufft_spread")):
+        if kname in kern_avg:
+            t = kern_avg[kname][0] * 1e-3
+            achieved = ab[kname] / t / 1e9
+            traffic = measured_traffic(args.workload, kname)
+            e = {"kernel": kname, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                 "traffic": round(traffic) if traffic else None, "algorithmic_bytes_per_launch": ab[kname],
+                 "avg_launch_ms": round(kern_avg[kname][0], 4)}
+            if kname == "nufft_spread":
+                useful = 8.0 * C * M * w ** 3
+                e["useful_tflops"] = round(useful / t / 1e12, 2)
+                e["frac_of_fp32_mfma_157TF"] = round(useful / t / 157.3e12, 4)
+            roof_nufft[tag] = e
     kernels_json = {}
     for k, (ms, n) in kern_avg.items():
         e = {"avg_ms": round(ms, 4), "launches": n}
@@ -615,10 +741,15 @@ def main():
         frames = bench_frames(dev)
 
     cpu = None
+    cpu_nufft = None
     if not args.no_cpu_baseline and "aha" in phases:
         log("[bench] CPU baseline ...")
         cpu = cpu_baseline(N, C, smaps_np, x_np, kernel.cpu().numpy(), max(1, min(args.cpu_sample_coils, C)))
         log(f"[bench] CPU baseline {cpu['value']:.4f} applies/s")
+    if not args.no_cpu_baseline and ("type1" in phases or "type2" in phases):
+        log("[bench] CPU NUFFT baseline ...")
+        cpu_nufft = cpu_baseline_nufft(N, C, coords_np, smaps_np, x_np, max(1, min(args.cpu_nufft_coils, C)))
+        log(f"[bench] CPU NUFFT baseline {cpu_nufft}")
 
     line = {
         "metric": "A^H A applies/s + NUFFT Mpts/s, 128^3x32 coils, <=2e-5 rel err vs FP64",
@@ -650,11 +781,14 @@ def main():
                                    "unit": "GB/s", "frac": round(ab["aha"] / aha_t / 1e9 / HBM_PEAK_GBS, 4),
                                    "algorithmic_bytes": ab["aha"]} if aha_t else None),
         "roofline": roof,
+        "roofline_nufft": roof_nufft or None,
         "kernels": kernels_json,
         "accuracy_rel_l2": acc,
+        "cg30": cg30,
         "psf_precompute_s": round(psf_s, 3),
         "setpts_s": round(setpts_s, 3),
         "cpu_baseline": cpu,
+        "cpu_baseline_nufft": cpu_nufft,
     }
     print(json.dumps(line), flush=True)
 

@@ -6,7 +6,9 @@
   Toeplitz kernel, and after 20 iterations the residual and the error vs the
   truth against FP64 CG's (the iterates themselves drift apart in FP32 on
   this ill-conditioned system; the difference is printed).
-* 128^3 x 32 coils x 8.4M points (the benchmark size), through properties
+* 128^3 x 32 coils x 8.4M points (the benchmark size): type 1, type 2 and
+  A^H A against FP64 references (direct sums, complex128 plans), a
+  30-iteration CG; and through properties
   that hold at any size: NUFFT adjointness <A f, c> = <f, A^H c>, the normal
   operator Hermitian and positive semi-definite, and the Toeplitz normal
   operator equal to (1/N^3) sum_c conj(S_c) A^H A S_c from the NUFFT pair.
@@ -132,6 +134,80 @@ def test_full_size_properties_128c32_koosh8M():
     p1.execute(Af, AHc)
     pair = (S.conj() * AHc).sum(0) / float(N ** 3)
     report("full size Toeplitz normal op vs NUFFT pair", rel_l2(host(Nv), host(pair)), 2e-5)
+
+
+def test_full_size_fp64_accuracy_128c32_koosh8M():
+    """The accuracy contract at the benchmark size (128^3 x 32 coils x 8.4M
+    points, eps 1e-5), against FP64 references:
+
+    * type 1 of every coil's data on 256 random voxels (coils 0, 13 and 31)
+      against a direct FP64 sum over all 8.4M points;
+    * type 2 on 512 random points of coils 0 and 31 against the FP64 NDFT;
+    * the whole A^H A apply (Toeplitz, c64) against complex128 NUFFT plans at
+      tol 1e-9, (1/N^3) sum_c conj(S_c) A1(A2(S_c x));
+    * 30 CG iterations on N x = b (FP64 scalars): the recurrence residual
+      agrees with the true residual ||b - N x|| of a fresh apply, and the
+      residual falls (both printed).
+    """
+    import bench
+    from hastycucompute_amd import fft as hf, synth
+    from hastycucompute_amd.mri import NORMAL_T_T1_OP
+    from hastycucompute_amd.op import conjugate_gradient
+    N, C, spokes, samples = 128, 32, 32768, 256
+    n = (N, N, N)
+    dev = torch.device("cuda", 0)
+    k = [torch.from_numpy(c).to(dev) for c in synth.kooshball(spokes, samples)]
+    M = spokes * samples
+    S = torch.from_numpy(synth.coil_maps(n, C, seed=1)).to(dev)
+    x = torch.from_numpy(synth.image(n, seed=0)).to(dev)
+    g = torch.Generator(device=dev).manual_seed(11)
+    c = torch.randn((C, M), dtype=torch.complex64, device=dev, generator=g)
+    p1 = hf.nufft_plan.make(hf.nufft_opts(nmodes=n, ntransf=C, tol=1e-5), hf.nufft_type.TYPE_1)
+    p1.setpts(k)
+    img = torch.empty((C,) + n, dtype=torch.complex64, device=dev)
+    p1.execute(c, img)
+    p1.free()
+    gc = torch.Generator().manual_seed(3)
+    vox = torch.stack([torch.randint(-(N // 2), (N + 1) // 2, (256,), generator=gc) for _ in range(3)], 1).to(dev)
+    vidx = tuple(vox[:, a] + N // 2 for a in range(3))
+    for cc in (0, 13, C - 1):
+        ref = bench.fp64_direct_type1_subset(c[cc], k, n, vox)
+        report(f"128^3 x 32 type 1 coil {cc} vs FP64 direct sum (256 voxels)",
+               float((img[cc][vidx].to(torch.complex128) - ref).norm() / ref.norm()), 2e-5)
+    del c
+    p2 = hf.nufft_plan.make(hf.nufft_opts(nmodes=n, ntransf=C, tol=1e-5), hf.nufft_type.TYPE_2)
+    p2.setpts(k)
+    f = (S * x[None]).contiguous()
+    y = torch.empty((C, M), dtype=torch.complex64, device=dev)
+    p2.execute(f, y)
+    p2.free()
+    idx = torch.randperm(M, generator=gc)[:512].to(dev)
+    for cc in (0, C - 1):
+        ref = bench.fp64_ndft_type2_subset(f[cc], k, idx)
+        report(f"128^3 x 32 type 2 coil {cc} vs FP64 NDFT (512 points)",
+               float((y[cc][idx].to(torch.complex128) - ref).norm() / ref.norm()), 2e-5)
+    del y, f, img
+    torch.cuda.empty_cache()
+    kernel = torch.empty((2 * N,) * 3, dtype=torch.complex64, device=dev)
+    hf.toeplitz_kernel(k, kernel)
+    op = NORMAL_T_T1_OP(kernel, S)
+    out = op(x)
+    ref_n = bench.fp64_plan_pair_normal(k, S, x, N, tol=1e-9)
+    report("128^3 x 32 A^H A vs complex128 NUFFT plans (tol 1e-9)",
+           float((out.to(torch.complex128) - ref_n).norm() / ref_n.norm()), 2e-5)
+    del ref_n
+    torch.cuda.empty_cache()
+    b = op(x)
+    xs = torch.zeros_like(x)
+    hist = []
+    conjugate_gradient(op, 30, 0.0)(xs, b, history=hist)
+    bn = float(b.to(torch.complex128).norm())
+    rec = float(hist[-1].item()) / bn
+    true = float((b - op(xs)).to(torch.complex128).norm()) / bn
+    print(f"[workload] 128^3 x 32, 30 CG iterations: residual {rec:.4e} (recurrence), {true:.4e} (true), "
+          f"first {float(hist[0].item()) / bn:.4e}", flush=True)
+    report("128^3 x 32 CG recurrence residual vs true residual", abs(rec - true) / true, 1e-3)
+    assert rec < float(hist[0].item()) / bn
 
 
 def test_trajectory_design_spiral_256c8():
