@@ -207,30 +207,48 @@ class _SenseCGAdjoint(torch.autograd.Function):
         op.set_coords(coords)
         nvox = float(torch.tensor(op.nmodes).prod())
         dev = y.device
-        kernel = torch.empty(tuple(2 * n for n in op.nmodes), dtype=torch.complex64, device=dev)
-        hf.toeplitz_kernel(list(coords), kernel, tol=kernel_tol)
-        N = NORMAL_T_T1_OP(kernel, smaps.contiguous())
+        N = _normal_operator(op, smaps, coords, kernel_tol, nvox)
         b = ((smaps.conj() * op.adjoint(y)).sum(0) / nvox).contiguous()
-        n = b.numel()
-        s = hf._stream(dev)
         x = torch.zeros_like(b)
         r = b.clone()
         p = torch.empty_like(b)
         q = torch.empty_like(b)
-        state = torch.zeros(CG_STATE_DOUBLES, dtype=torch.float64, device=dev)
-        check(lib().hasty_cg_init(r.data_ptr(), p.data_ptr(), n, 0.0, state.data_ptr(), s))
         K = int(iters)
         P, Q, R = [], [], []
         scal = torch.zeros((K, 5), dtype=torch.float64, device=dev)   # rz_i, pq_i, alpha_i, rz_i+1, beta_i
-        for i in range(K):
-            P.append(p.clone())
-            scal[i, 0] = state[0]
-            N.apply(p, q)
-            Q.append(q.clone())
-            check(lib().hasty_cg_step(x.data_ptr(), r.data_ptr(), p.data_ptr(), q.data_ptr(), n,
-                                      state.data_ptr(), s))
-            scal[i, 1:5] = state[1:5]
-            R.append(r.clone())
+        if b.dtype == torch.complex64:
+            # the library's CG step: FP64 scalars on the device, FP32 vectors
+            n = b.numel()
+            s = hf._stream(dev)
+            state = torch.zeros(CG_STATE_DOUBLES, dtype=torch.float64, device=dev)
+            check(lib().hasty_cg_init(r.data_ptr(), p.data_ptr(), n, 0.0, state.data_ptr(), s))
+            for i in range(K):
+                P.append(p.clone())
+                scal[i, 0] = state[0]
+                N.apply(p, q)
+                Q.append(q.clone())
+                check(lib().hasty_cg_step(x.data_ptr(), r.data_ptr(), p.data_ptr(), q.data_ptr(), n,
+                                          state.data_ptr(), s))
+                scal[i, 1:5] = state[1:5]
+                R.append(r.clone())
+        else:
+            # complex128: the same recurrence in FP64 throughout
+            p.copy_(r)
+            rz = _rdot64(r, r)
+            for i in range(K):
+                P.append(p.clone())
+                N.apply(p, q)
+                Q.append(q.clone())
+                pq = _rdot64(p, q)
+                alpha = rz / pq
+                x += alpha * p
+                r -= alpha * q
+                rz1 = _rdot64(r, r)
+                beta = rz1 / rz
+                p.mul_(beta).add_(r)
+                scal[i] = torch.stack([rz, pq, alpha, rz1, beta])
+                R.append(r.clone())
+                rz = rz1
         ctx.op, ctx.N, ctx.nvox = op, N, nvox
         ctx.P, ctx.Q, ctx.R, ctx.scal = P, Q, R, scal
         ctx.save_for_backward(y, smaps, b, *coords)
@@ -244,7 +262,7 @@ class _SenseCGAdjoint(torch.autograd.Function):
         want_k = any(ctx.needs_input_grad[5:])
         P, Q, R, scal = ctx.P, ctx.Q, ctx.R, ctx.scal
         K = len(P)
-        f32 = lambda t: t.to(torch.float32)     # noqa: E731  (FP64 device scalar -> FP32 factor)
+        f32 = lambda t: t.to(gx.real.dtype)     # noqa: E731  (FP64 device scalar -> the vectors' real type)
         gx = gx.contiguous()
         gr = torch.zeros_like(gx)
         gp = torch.zeros_like(gx)
@@ -301,11 +319,37 @@ class _SenseCGAdjoint(torch.autograd.Function):
 
 
 def _rdot64(a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
-    """Re <a, b> accumulated in FP64 on the device (0-d float64 tensor)."""
-    out = torch.empty(2, dtype=torch.float64, device=a.device)
+    """Re <a, b> accumulated in FP64 on the device (0-d float64 tensor): the
+    library's FP64 reduction for complex64, torch's for complex128."""
     aa, bb = a.contiguous(), b.contiguous()
+    if aa.dtype == torch.complex128:
+        return torch.vdot(aa.reshape(-1), bb.reshape(-1)).real
+    out = torch.empty(2, dtype=torch.float64, device=a.device)
     check(lib().hasty_dot(aa.data_ptr(), bb.data_ptr(), aa.numel(), out.data_ptr(), hf._stream(a.device)))
     return out[0]
+
+
+class _PairNormal:
+    """N x = (1/N^d) sum_c conj(S_c) A^H A (S_c x) through the NUFFT pair of a
+    complex128 operator (the Toeplitz passes are complex64)."""
+
+    def __init__(self, op, smaps, nvox):
+        self.op, self.smaps, self.nvox = op, smaps.contiguous(), nvox
+
+    def apply(self, x, out):
+        Sx = (self.smaps * x.unsqueeze(0)).contiguous()
+        out.copy_((self.smaps.conj() * self.op.adjoint(self.op.forward(Sx))).sum(0) / self.nvox)
+        return out
+
+
+def _normal_operator(op, smaps, coords, kernel_tol, nvox):
+    """The normal operator of the solve: the Toeplitz NORMAL_T_T1_OP (FP64 PSF)
+    for complex64 operators, the NUFFT pair for complex128 ones."""
+    if op.cdtype == torch.complex128:
+        return _PairNormal(op, smaps, nvox)
+    kernel = torch.empty(tuple(2 * n for n in op.nmodes), dtype=torch.complex64, device=smaps.device)
+    hf.toeplitz_kernel(list(coords), kernel, tol=kernel_tol)
+    return NORMAL_T_T1_OP(kernel, smaps.contiguous())
 
 
 def sense_cg_solve(y: torch.Tensor, smaps: torch.Tensor, op: NufftOperator, *coords: torch.Tensor,
@@ -316,7 +360,11 @@ def sense_cg_solve(y: torch.Tensor, smaps: torch.Tensor, op: NufftOperator, *coo
     ``method="adjoint"`` (default) differentiates the truncated solve exactly,
     in reverse mode through the checkpointed iterates; ``method="implicit"``
     returns the gradient of the converged solution (one adjoint CG solve,
-    valid once the forward solve has converged)."""
+    valid once the forward solve has converged).  A complex128 operator
+    (``NufftOperator(dtype=torch.float64)``) runs the adjoint method in FP64
+    throughout, on the normal operator of its NUFFT pair."""
+    if method == "implicit" and op.cdtype != torch.complex64:
+        raise RuntimeError("method 'implicit' needs a complex64 operator")
     if method == "adjoint":
         return _SenseCGAdjoint.apply(y, smaps, op, iters, kernel_tol, *coords)
     if method == "implicit":

@@ -88,10 +88,15 @@ def _problem(seed, shape=(32, 32), C=4, M=2000):
     return coords, x, S, y
 
 
-def test_autograd_nufft2_vs_fp64_ndft_and_fd():
+@pytest.mark.parametrize("shape,M", [((32, 32), 2000), ((64, 64), 4096)])
+def test_autograd_nufft2_vs_fp64_ndft_and_fd(shape, M):
+    """The trajectory-design loss at 32^2 and at 64^2 (the size the
+    differentiable-NUFFT workload checks against autograd through an FP64
+    NDFT): image and trajectory gradients vs FP64 autograd, trajectory
+    gradient vs FP64 central differences (h = 1e-4)."""
     from hastycucompute_amd import autograd as ag
-    shape, C = (32, 32), 4
-    coords, x, S, y = _problem(13, shape, C)
+    C = 4
+    coords, x, S, y = _problem(13, shape, C, M)
     op = ag.NufftOperator(shape, ntransf=C, tol=1e-6)
     k = [cuda(c).requires_grad_(True) for c in coords]
     xg = cuda(x).requires_grad_(True)
@@ -104,10 +109,10 @@ def test_autograd_nufft2_vs_fp64_ndft_and_fd():
     L = _ref_loss(x64, S64, y64, kk, shape)
     L.backward()
     lv, Lv = float(loss.detach()), float(L.detach())
-    report("loss vs FP64 NDFT", abs(lv - Lv) / abs(Lv), 1e-5)
-    report("d loss / d image vs FP64 autograd", rel_l2(host(xg.grad), x64.grad.numpy()), 1e-4)
+    report(f"{shape} loss vs FP64 NDFT", abs(lv - Lv) / abs(Lv), 1e-5)
+    report(f"{shape} d loss / d image vs FP64 autograd", rel_l2(host(xg.grad), x64.grad.numpy()), 1e-4)
     for a in range(2):
-        report(f"d loss / d k[{a}] vs FP64 autograd", rel_l2(host(k[a].grad), kk[a].grad.numpy()), 1e-4)
+        report(f"{shape} d loss / d k[{a}] vs FP64 autograd", rel_l2(host(k[a].grad), kk[a].grad.numpy()), 1e-4)
     # central finite differences of the FP64 loss along random directions
     rng = np.random.default_rng(99)
     g = np.concatenate([host(k[a].grad) for a in range(2)]).astype(np.float64)
@@ -123,7 +128,7 @@ def test_autograd_nufft2_vs_fp64_ndft_and_fd():
             fd = (float(_ref_loss(x64.detach(), S64, y64, kp, shape)) -
                   float(_ref_loss(x64.detach(), S64, y64, km, shape))) / (2 * hstep)
             errs.append(abs(float(g @ dvec) - fd) / np.linalg.norm(g))
-    report("d loss / d k vs central FD (8 directions, max rel)", max(errs), 1e-4)
+    report(f"{shape} d loss / d k vs central FD (8 directions, max rel)", max(errs), 1e-4)
 
 
 def test_autograd_nufft1_vs_fp64():

@@ -15,10 +15,11 @@
 * Trajectory design at 256^2 x 8 coils, 16 spiral interleaves x 4096
   samples: 200 Adam steps on the trajectory lower the data loss.
 * Gradient through the SENSE-CG solve at 160^3 x 16 coils, stack of spirals
-  40 x 24 x 2048 (1.97M points), 25 CG iterations (timed): the reverse-mode
-  adjoint through the checkpointed iterates agrees with central differences
-  of the same pipeline (FP64 loss sums) along 8 random trajectory directions
-  to 2%, the level the FP32 pipeline's rounding leaves in the differences.
+  40 x 24 x 2048 (1.97M points), 25 CG iterations: the reverse-mode adjoint
+  through the checkpointed iterates of the FP64 pipeline (complex128 plans,
+  FP64 CG) agrees with FP64 central differences along 8 random trajectory
+  directions to 1e-4; the production FP32 pipeline (timed) agrees with the
+  same differences to 1e-3.
 """
 import numpy as np
 import pytest
@@ -231,6 +232,19 @@ def test_trajectory_design_spiral_256c8():
 
 
 def test_cg_gradient_160c16_stack_of_spirals():
+    """cfg5: the gradient of L(k) = ||x_25(k) - x_true||^2 / ||x_true||^2 through
+    25 CG iterations, 160^3 x 16 coils, stack of spirals 40 x 24 x 2048.
+
+    * The FP64 pipeline (complex128 NUFFT plans at tol 1e-8, FP64 CG on the
+      normal operator of the NUFFT pair): the reverse-mode adjoint through the
+      checkpointed iterates against FP64 central differences of the same
+      pipeline (h = 1e-2 along unit directions in trajectory space), on 8
+      random directions: <= 1e-4 of the RMS directional derivative.
+    * The production FP32 pipeline (Toeplitz normal operator at tol 1e-5,
+      FP64 CG scalars; timed): its adjoint against the same FP64 central
+      differences, <= 1e-3 (measured 3e-4: FP32 rounding in the 25 CG steps
+      and the tolerance gap between the two pipelines).
+    """
     import time
     from hastycucompute_amd import autograd as ag, synth
     dev = torch.device("cuda", 0)
@@ -240,19 +254,56 @@ def test_cg_gradient_160c16_stack_of_spirals():
     M = int(k[0].shape[0])
     S = torch.from_numpy(synth.coil_maps(n, C, seed=1)).to(dev)
     x_true = torch.from_numpy(synth.image(n, seed=0)).to(dev)
+    xt64 = x_true.to(torch.complex128)
+    # ---- FP64 pipeline: adjoint gradient and central differences ----
+    op64 = ag.NufftOperator(n, ntransf=C, tol=1e-8, dtype=torch.float64)
+    k64 = [c.double() for c in k]
+    S64 = S.to(torch.complex128)
+    with torch.no_grad():
+        y64 = ag.sense_forward(xt64, S64, op64, *k64).contiguous()
+
+    def loss64(kk):
+        xh = ag.sense_cg_solve(y64, S64, op64, *kk, iters=iters)
+        return (xh - xt64).abs().pow(2).sum() / xt64.abs().pow(2).sum()
+
+    kg64 = [c.clone().requires_grad_(True) for c in k64]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    L64 = loss64(kg64)
+    L64.backward()
+    torch.cuda.synchronize()
+    g64 = [c.grad for c in kg64]
+    print(f"[workload] 160^3 x 16 coils x {M} pts FP64 pipeline: loss {float(L64.detach()):.6e}, "
+          f"forward + backward {time.perf_counter() - t0:.1f} s", flush=True)
+    gen = torch.Generator(device=dev).manual_seed(8)
+    dirs, fds = [], []
+    h = 1e-2
+    for _ in range(8):
+        d = [torch.randn(c.shape, device=dev, dtype=torch.float64, generator=gen) for c in k]
+        nrm = float(torch.sqrt(sum((di ** 2).sum() for di in d)))
+        d = [di / nrm for di in d]
+        with torch.no_grad():
+            lp = float(loss64([c + h * di for c, di in zip(k64, d)]))
+            lm = float(loss64([c - h * di for c, di in zip(k64, d)]))
+        dirs.append(d)
+        fds.append((lp - lm) / (2 * h))
+    an64 = [float(sum((gi * di).sum() for gi, di in zip(g64, d))) for d in dirs]
+    scale = float(np.sqrt(np.mean(np.square(fds))))
+    for a_, f_ in zip(an64, fds):
+        print(f"[workload]   direction: FP64 adjoint {a_:+.9e}  FP64 central difference {f_:+.9e}", flush=True)
+    report("160^3 FP64 truncated-CG adjoint vs FP64 central differences (8 directions, max |err| / rms)",
+           max(abs(a_ - f_) for a_, f_ in zip(an64, fds)) / scale, 1e-4)
+    del op64, kg64, g64, y64
+    torch.cuda.empty_cache()
+    # ---- FP32 pipeline (production): timed, against the FP64 differences ----
     op = ag.NufftOperator(n, ntransf=C, tol=1e-5)
     with torch.no_grad():
         y = ag.sense_forward(x_true, S, op, *k).contiguous()
-    xt64 = x_true.to(torch.complex128)
-
-    def loss_of(kk):
-        xh = ag.sense_cg_solve(y, S, op, *kk, iters=iters)
-        return (xh.to(torch.complex128) - xt64).abs().pow(2).sum() / xt64.abs().pow(2).sum()
-
     kg = [c.clone().requires_grad_(True) for c in k]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    L = loss_of(kg)
+    xh = ag.sense_cg_solve(y, S, op, *kg, iters=iters)
+    L = (xh.to(torch.complex128) - xt64).abs().pow(2).sum() / xt64.abs().pow(2).sum()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     L.backward()
@@ -260,26 +311,10 @@ def test_cg_gradient_160c16_stack_of_spirals():
     t2 = time.perf_counter()
     g = [c.grad.double() for c in kg]
     assert all(torch.isfinite(gi).all() for gi in g)
-    print(f"[workload] 160^3 x 16 coils x {M} pts, {iters} CG iterations: forward {t1 - t0:.2f} s "
-          f"(incl. PSF), adjoint backward {t2 - t1:.2f} s, loss {float(L.detach()):.4e}", flush=True)
-    # central differences of the same pipeline along 8 random unit directions
-    # in trajectory space (h = 1 moves each point by ~4e-4 rad on average)
-    gen = torch.Generator(device=dev).manual_seed(8)
-    pairs = []
-    for _ in range(8):
-        d = [torch.randn(c.shape, device=dev, dtype=torch.float64, generator=gen) for c in k]
-        nrm = float(torch.sqrt(sum((di ** 2).sum() for di in d)))
-        d = [di / nrm for di in d]
-        h = 1.0
-        with torch.no_grad():
-            lp = float(loss_of([(c.double() + h * di).float() for c, di in zip(k, d)]))
-            lm = float(loss_of([(c.double() - h * di).float() for c, di in zip(k, d)]))
-        fd = (lp - lm) / (2 * h)
-        an = float(sum((gi * di).sum() for gi, di in zip(g, d)))
-        pairs.append((an, fd))
-        print(f"[workload]   direction: adjoint {an:+.6e}  central difference {fd:+.6e}", flush=True)
-    # errors relative to the typical directional derivative (a direction
-    # whose derivative happens to be near zero is judged on the same scale)
-    scale = float(np.sqrt(np.mean([fd * fd for _, fd in pairs])))
-    report("160^3 truncated-CG gradient vs central differences (8 random directions, max |err| / rms)",
-           max(abs(an - fd) for an, fd in pairs) / scale, 2e-2)
+    print(f"[workload] 160^3 x 16 coils x {M} pts FP32 pipeline, {iters} CG iterations: forward {t1 - t0:.2f} s "
+          f"(incl. PSF), adjoint backward {t2 - t1:.2f} s, loss {float(L.detach()):.6e}", flush=True)
+    an32 = [float(sum((gi * di).sum() for gi, di in zip(g, d))) for d in dirs]
+    for a_, f_ in zip(an32, fds):
+        print(f"[workload]   direction: FP32 adjoint {a_:+.6e}  FP64 central difference {f_:+.6e}", flush=True)
+    report("160^3 FP32 truncated-CG adjoint vs FP64 central differences (8 directions, max |err| / rms)",
+           max(abs(a_ - f_) for a_, f_ in zip(an32, fds)) / scale, 1e-3)
