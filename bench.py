@@ -320,7 +320,7 @@ _SQ_PASSES = ("SQ_INSTS_VALU SQ_INSTS_SALU SQ_INSTS_SMEM SQ_INSTS_VMEM_RD SQ_WAI
               "SQ_BUSY_CYCLES SQ_ACTIVE_INST_VALU",
               "SQ_WAVES SQ_INSTS_VALU_FMA_F32 SQ_INSTS_LDS SQ_LDS_BANK_CONFLICT SQ_ACTIVE_INST_LDS GRBM_GUI_ACTIVE")
 _SQ_KERNELS = (("spread_kernel_w6<c64, 32>", "spread_kernel_w6<HIP_vector_type<float, 2u>, 32>", 20_000_000),
-               ("spread_lanes32_kernel<6>", "spread_lanes32_kernel<6>", 10_000_000),
+               ("spread_lanes_kernel<6, 32>", "spread_lanes_kernel<6, 32>", 10_000_000),
                ("interp_pair_kernel<6, 16, 16, 16>", "interp_pair_kernel<6, 16, 16, 16>", 40_000_000))
 
 
@@ -431,25 +431,34 @@ def bench_frames(dev, iters=15, nframes=20, N=96, C=16, samples=192, seed=3, rep
     fb = FrameBatch(S, coords, kdata, tol=1e-5, nufft_opts=autotune.plan_options(tuned))
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t0
+    # one solve = every frame's right-hand side (type-1 NUFFT of its data) and
+    # `iters` CG iterations; eager one frame after another on one stream, eager
+    # on the concurrent branches, and the whole solve replayed from one graph
+    modes = {"eager": dict(graph=False, branches=False), "eager_branches": dict(graph=False, branches=True),
+             "graph": dict(graph=True)}
     out = {}
-    for mode in ("eager", "graph"):
-        fb.solve(iters, graph=(mode == "graph"))       # warm (and capture)
+    for mode, kw in modes.items():
+        fb.solve(iters, **kw)       # warm (and capture)
         torch.cuda.synchronize()
         t = time.perf_counter()
         for _ in range(reps):
-            fb.solve(iters, graph=(mode == "graph"))
+            fb.solve(iters, **kw)
         torch.cuda.synchronize()
         out[mode] = (time.perf_counter() - t) / reps
-    xe = fb.solve(iters, graph=False).clone()
+    xe = fb.solve(iters, graph=False, branches=False).clone()
     xg = fb.solve(iters, graph=True).clone()
     same = bool(torch.equal(xe, xg))
     total_pts = int(sum(int(c[0].shape[0]) for c in coords))
     log(f"[bench] frames: {nframes} x {N}^3 x {C} coils, {total_pts} pts, {iters} CG iters: "
-        f"eager {out['eager'] * 1e3:.1f} ms, graph {out['graph'] * 1e3:.1f} ms, identical={same}")
+        f"eager {out['eager'] * 1e3:.1f} ms, eager branches {out['eager_branches'] * 1e3:.1f} ms, "
+        f"graph {out['graph'] * 1e3:.1f} ms, identical={same}")
     return {"frames": nframes, "image": [N] * 3, "coils": C, "cg_iters": iters,
             "spokes_per_frame": [int(s) for s in spokes], "samples_per_spoke": samples,
             "total_points_per_coil": total_pts,
+            "solve": "right-hand sides (type-1 NUFFT per frame) + CG, every frame",
+            "streams": fb.nstreams,
             "eager_ms_per_solve": round(out["eager"] * 1e3, 3),
+            "eager_branches_ms_per_solve": round(out["eager_branches"] * 1e3, 3),
             "graph_ms_per_solve": round(out["graph"] * 1e3, 3),
             "graph_speedup": round(out["eager"] / out["graph"], 3),
             "frame_iterations_per_s": round(nframes * iters / out["graph"], 1),

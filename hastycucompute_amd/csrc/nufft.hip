@@ -857,19 +857,19 @@ using SpreadFn = void (*)(GridArgs<V>, const int4*, const V*, const decltype(V::
 // 32 tiles (a row of tiles along x shares its straddling points in one L2:
 // 28.0 -> 24.7 ms); interpolation is fastest with single subproblems
 // interleaved over the XCDs.  HASTY_XCD_CHUNK overrides both (tuning).
-// The lanes-as-transforms spreading kernel, when it applies: complex64 3-D
-// plans spreading 32 transforms per block, an even batch (16-B aligned value
-// rows), every fine-grid axis >= w + 4 (a support meets a 4-cell sub-block in
-// one piece).  HASTY_SPREAD_LANES=0 keeps the cell-per-lane kernel.
+// The MFMA spreading kernel, when it applies: complex64 3-D plans spreading
+// 16 or 32 transforms per block, an even batch (16-B aligned value rows),
+// every fine-grid axis >= w + 4 (a support meets a 4-cell sub-block in one
+// piece).  HASTY_SPREAD_LANES=0 keeps the cell-per-lane kernel.
 static void* spread_lanes_fn(const NufftPlan* p, int nb) {
     static const bool off = [] {
         const char* e = std::getenv("HASTY_SPREAD_LANES");
         return e && e[0] == '0';
     }();
-    if (off || p->f64 || p->dim != 3 || p->group != 32 || nb % 2 != 0) return nullptr;
+    if (off || p->f64 || p->dim != 3 || (p->group != 32 && p->group != 16) || nb % 2 != 0) return nullptr;
     for (int a = 0; a < 3; ++a)
         if (p->nf[a] < p->w + 4) return nullptr;
-    return nufft_spread_lanes32_ptr(p->w);
+    return nufft_spread_lanes_ptr(p->w, p->group);
 }
 
 static int xcd_chunk(bool spread) {
@@ -992,7 +992,7 @@ void execute_impl(NufftPlan* p, V* c, V* f, const NufftHooks& hk, int ntransf) {
                 ga.subs = p->sub_spread;
                 if (void* lanes_fn = spread_lanes_fn(p, nb)) {
                     HASTY_PROF("nufft_spread", s);
-                    const int ng = int(ceil_div(nb, 32));
+                    const int ng = int(ceil_div(nb, p->group));
                     ga.nsub = p->nsub_spread;
                     ga.ngroups = ng;
                     ga.xchunk = xcd_chunk(true);

@@ -30,16 +30,14 @@ template <> void* nufft_spread_ptr<float2>(int cg) {
     throw Error(HASTY_ERR_INTERNAL, "no spreading kernel for this transform group");
 }
 
-void* nufft_spread_lanes32_ptr(int w) {
-    switch (w) {
-        case 2: return reinterpret_cast<void*>(&spread_lanes32_kernel<2>);
-        case 3: return reinterpret_cast<void*>(&spread_lanes32_kernel<3>);
-        case 4: return reinterpret_cast<void*>(&spread_lanes32_kernel<4>);
-        case 5: return reinterpret_cast<void*>(&spread_lanes32_kernel<5>);
-        case 6: return reinterpret_cast<void*>(&spread_lanes32_kernel<6>);
-        case 7: return reinterpret_cast<void*>(&spread_lanes32_kernel<7>);
-        case 8: return reinterpret_cast<void*>(&spread_lanes32_kernel<8>);
-        case 9: return reinterpret_cast<void*>(&spread_lanes32_kernel<9>);
+void* nufft_spread_lanes_ptr(int w, int group) {
+    switch (w * 64 + group) {
+#define HASTY_CASE(Wv)                                                                          \
+    case Wv * 64 + 16: return reinterpret_cast<void*>(&spread_lanes_kernel<Wv, 16>);           \
+    case Wv * 64 + 32: return reinterpret_cast<void*>(&spread_lanes_kernel<Wv, 32>);
+        HASTY_CASE(2) HASTY_CASE(3) HASTY_CASE(4) HASTY_CASE(5) HASTY_CASE(6) HASTY_CASE(7)
+        HASTY_CASE(8) HASTY_CASE(9)
+#undef HASTY_CASE
     }
     return nullptr;
 }

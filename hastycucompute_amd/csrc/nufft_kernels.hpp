@@ -446,21 +446,25 @@ __global__ void __launch_bounds__(kNufftThreads) __attribute__((amdgpu_waves_per
 }
 
 // ---------------------------------------------------------------------------
-// Spreading with the transforms on the lanes (complex64, 32 transforms per
-// block, 3-D).  Lane l of a wave holds float l of a point's 32 complex values
-// (transform l / 2, real or imaginary part), so the whole wave works on one
-// point at a time and every weight is wave-uniform: a (z, y) row of the
-// wave's 4 x 4 x 4 sub-block that the point's support misses is skipped by a
-// scalar branch, so only in-support rows do work (1 + 4 FMA-class ops; x
-// cells outside the support take zero weights).  Each lane keeps 64 FP32
-// accumulators: one cell of the sub-block each, for its transform and part.
+// Spreading on the matrix cores (complex64, 16 or 32 transforms per block,
+// 3-D).  For one wave's 4 x 4 x 4 sub-block of a tile, the sum over the
+// points that touch it is a rank-K update
+//   acc[tp][cell] += sum_hits value[hit][tp] * weight[hit][cell],
+// tp = 2 transform + part (real, imaginary), cell = 16 z + 4 y + x, so it runs
+// as FP32 MFMAs: v_mfma_f32_32x32x2_f32 takes the hits two at a time (lanes
+// 0-31 carry hit k = 0, lanes 32-63 hit k = 1 of the pair; an odd last hit
+// pairs with zero weights), A = the hits' values (32 transform-parts per
+// tile), B = their window weights w_z w_y w_x at the 64 cells (two 32-cell
+// tiles).  Each lane keeps 16 FP32 accumulators per tile, 32 or 64 in all.
+// Cells outside a support take zero weights; that costs matrix work but no
+// branches.  MFMA products and sums are FP32.
 //
 // The block (8 waves, one 4^3 sub-block each; larger tiles take several
-// passes) walks the tile's visit list in chunks of 64 records.  Every chunk
-// is staged once in LDS for all 8 waves: the records, the points' 32 complex
-// values (256 B each) and their window weights as zero-padded rows (index
+// passes) walks the tile's visit list in chunks of 128 records.  Every chunk
+// is staged once in LDS for all 8 waves: the records, the points' values
+// (2 NTG floats each) and their window weights in zero-padded rows (index
 // q + 3 for q = cell - first support cell, so any cell of a sub-block reads
-// a weight or a zero).  Each wave tests the 64 records against its sub-block
+// a weight or a zero).  Each wave tests the records against its sub-block
 // and takes the hits in visit order, so every cell's sum has a fixed order
 // and the result is the same on every run.  No atomics.  Requires every axis
 // of the fine grid to be >= w + 4 (a support then meets a sub-block in one
@@ -478,9 +482,9 @@ __device__ __forceinline__ int lane_hit_pop(unsigned long long& mask) {
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
-template <int W>
+template <int W, int NTG>
 __global__ void __launch_bounds__(kLaneSpreadThreads)
-    spread_lanes32_kernel(GridArgs<float2> a, const int4* __restrict__ visits,
+    spread_lanes_kernel(GridArgs<float2> a, const int4* __restrict__ visits,
                           const float2* __restrict__ csort, const float* __restrict__ sw) {
     static_assert(W + 6 <= kLaneSpreadPad, "padded weight row too short");
     static_assert(kLaneSpreadChunk % 64 == 0 && kLaneSpreadThreads == 512, "staging: 8 threads per record");
@@ -498,8 +502,10 @@ __global__ void __launch_bounds__(kLaneSpreadThreads)
     const int n0 = int(a.g.nf[0]), n1 = int(a.g.nf[1]), n2 = int(a.g.nf[2]);
     const int ns1 = (T1 + 3) / 4, ns2 = (T2 + 3) / 4;
     const int nsb = ((T0 + 3) / 4) * ns1 * ns2;
-    const int g0 = igroup * 32;
-    const int ng = min(32, a.nbatch - g0);
+    static_assert(NTG == 16 || NTG == 32, "16 or 32 transforms per block");
+    constexpr int MT = NTG / 16;                    // 32-row MFMA tiles of transform-parts
+    const int g0 = igroup * NTG;
+    const int ng = min(NTG, a.nbatch - g0);
     const int lane = int(threadIdx.x) & 63;
     const int wave = wave_uniform(int(threadIdx.x >> 6));
     const int64_t G = a.g.nf[0] * a.g.nf[1] * a.g.nf[2];
@@ -517,9 +523,9 @@ __global__ void __launch_bounds__(kLaneSpreadThreads)
         const int G0 = t0[0] + o0, G1 = t0[1] + o1, G2 = t0[2] + o2;   // sub-block origin
         // acc[m][n]: the 32 x 32 MFMA tile of transform-parts 32 m .. 32 m + 31
         // by sub-block cells 32 n .. 32 n + 31 (cell = 16 z + 4 y + x)
-        floatx16 acc[2][2];
+        floatx16 acc[MT][2];
 #pragma unroll
-        for (int m = 0; m < 2; ++m)
+        for (int m = 0; m < MT; ++m)
 #pragma unroll
             for (int n = 0; n < 2; ++n)
 #pragma unroll
@@ -541,8 +547,13 @@ __global__ void __launch_bounds__(kLaneSpreadThreads)
                     const int4 rec = nrec_pf[rb / 64];
                     const int s = rec.x >> 3;
                     if (j == 0) lrec[r] = rec;
+                    // the point's 2 NTG floats: 8 per thread, threads j < NTG / 4
                     const float4* src = reinterpret_cast<const float4*>(csort + int64_t(s) * a.nbatch + g0) + 2 * j;
-                    const float4 u0 = src[0], u1 = src[1];
+                    float4 u0 = make_float4(0.f, 0.f, 0.f, 0.f), u1 = u0;
+                    if (j < NTG / 4) {
+                        u0 = src[0];
+                        u1 = src[1];
+                    }
                     // the 3W weights (rows padded to 16 B in sw): float4 j < ceil(3W/4)
                     float4 wq = make_float4(0.f, 0.f, 0.f, 0.f);
                     if (4 * j < 3 * W) wq = reinterpret_cast<const float4*>(sw + int64_t(s) * a.ws)[j];
@@ -597,7 +608,7 @@ __global__ void __launch_bounds__(kLaneSpreadThreads)
                     const int pk = __builtin_amdgcn_readlane(packed, i0) * (1 - k) +
                                    (i1 >= 0 ? __builtin_amdgcn_readlane(packed, i1) : 0) * k;
                     const float* vr = lval + (rb + ih) * 64;
-                    const float a0 = vr[j], a1 = vr[32 + j];
+                    const float a0 = vr[j], a1 = MT == 2 ? vr[32 + j] : 0.f;
                     const float* wr = lw + (rb + ih) * 3 * PW;
                     const float wzr0 = wr[(pk & 15) + cz], wzr1 = wr[(pk & 15) + cz + 2];
                     const float wyx = wr[PW + ((pk >> 4) & 15) + cy] * wr[2 * PW + ((pk >> 8) & 15) + cx];
@@ -605,8 +616,10 @@ __global__ void __launch_bounds__(kLaneSpreadThreads)
                     const float b0 = live ? wzr0 * wyx : 0.f, b1 = live ? wzr1 * wyx : 0.f;
                     acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, acc[0][0], 0, 0, 0);
                     acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b1, acc[0][1], 0, 0, 0);
-                    acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b0, acc[1][0], 0, 0, 0);
-                    acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, acc[1][1], 0, 0, 0);
+                    if constexpr (MT == 2) {
+                        acc[MT - 1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b0, acc[MT - 1][0], 0, 0, 0);
+                        acc[MT - 1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, acc[MT - 1][1], 0, 0, 0);
+                    }
                 }
             }
             __syncthreads();
@@ -620,7 +633,7 @@ __global__ void __launch_bounds__(kLaneSpreadThreads)
         int lw_ = lane;
         asm volatile("" : "+v"(lw_));
 #pragma unroll
-        for (int m = 0; m < 2; ++m)
+        for (int m = 0; m < MT; ++m)
 #pragma unroll
             for (int n = 0; n < 2; ++n)
 #pragma unroll
@@ -929,8 +942,8 @@ template <class V> void* nufft_interp_fixed_ptr(int dim, int w);
 void* nufft_interp_pair_ptr(int dim, int w);
 // spreading kernel for CG transforms per block (1, 2, 4, 8, 16, 32 / 16 for complex128)
 template <class V> void* nufft_spread_ptr(int cg);
-// complex64 spreading with the 32 transforms of a block on the lanes
-// (spread_lanes32_kernel), or nullptr for widths without an instance
-void* nufft_spread_lanes32_ptr(int w);
+// complex64 spreading with the 16 or 32 transforms of a block on the MFMA
+// rows (spread_lanes_kernel), or nullptr for widths without an instance
+void* nufft_spread_lanes_ptr(int w, int group);
 
 }  // namespace hasty

@@ -3,16 +3,24 @@
 
 Frame f has trajectory k_f (M_f points), its own Toeplitz PSF kernel K_f and
 SENSE normal operator N_f x = (1/N^d) sum_c conj(S_c) A_f^H A_f (S_c x)
-(``NORMAL_T_T1_OP``, all coils per launch, coil maps shared by the frames).
-The right-hand sides b_f = (1/N^d) sum_c conj(S_c) A_f^H y_f come from one
-all-coil type-1 NUFFT per frame.
+(``NORMAL_T_T1_OP``, all coils per launch, coil maps shared by the frames),
+and its own type-1 plan with its points registered once.  A solve is, per
+frame, the right-hand side b_f = (1/N^d) sum_c conj(S_c) A_f^H y_f (one
+all-coil type-1 NUFFT with conj(S_c) and the coil sum fused into its last
+FFT pass), then ``iters`` CG iterations from x = 0, every CG scalar on the
+device.
 
-CG runs on every frame at once.  One iteration of all frames -- F normal
-operator applies and F fused CG updates, every scalar on the device -- is
-captured once into a HIP graph (``torch.cuda.CUDAGraph``) and replayed, so
-the host issues one graph launch per iteration and never synchronises
-between iterations.  The eager path runs the same launches in the same
-order; both give identical results (every kernel is deterministic).
+The frames share nothing, so a solve runs them as ``streams`` independent
+branches: frame f on branch f mod streams, each branch on its own HIP stream
+(default priority), joined at the end.  At 96^3 x 16 coils one frame's
+launches do not fill the GPU (the fused last Toeplitz pass has 288 blocks of
+one per CU), and the branches fill it.  ``solve(graph=True)`` captures the
+whole solve -- right-hand sides, CG initialisation and every iteration of
+every frame, all branches -- once into a HIP graph (``torch.cuda.CUDAGraph``)
+and replays it as one launch.  The eager path runs the same launches in the
+same order per frame, on one stream (``streams=1``) or on the branches; all
+of them give identical results, bit for bit (every kernel is deterministic
+and no frame reads another's data).
 """
 from __future__ import annotations
 
@@ -28,11 +36,13 @@ from .mri import NORMAL_T_T1_OP
 class FrameBatch:
     def __init__(self, smaps: torch.Tensor, coords: Sequence[Sequence[torch.Tensor]],
                  kdata: Sequence[torch.Tensor], tol: float = 1e-5, kernel_tol: float = 1e-9,
-                 nufft_opts: Optional[Dict] = None):
+                 nufft_opts: Optional[Dict] = None, streams: int = 4):
         if smaps.dtype != torch.complex64 or smaps.device.type != "cuda":
             raise RuntimeError("smaps must be a complex64 GPU tensor [C, N...]")
         if len(coords) != len(kdata) or not coords:
             raise RuntimeError("one trajectory and one k-space array per frame")
+        if streams < 1:
+            raise RuntimeError("streams must be >= 1")
         self.F = len(coords)
         self.C = int(smaps.shape[0])
         self.n = tuple(int(s) for s in smaps.shape[1:])
@@ -43,78 +53,116 @@ class FrameBatch:
         self.device = dev
         self.smaps = smaps.contiguous()
         opts = dict(nufft_opts or {})
-        adj = hf.nufft_plan.make(hf.nufft_opts(nmodes=self.n, ntransf=self.C, tol=tol,
-                                               sign=hf.nufft_sign.DEFAULT_TYPE_1, **opts),
-                                 hf.nufft_type.TYPE_1)
         self.kernels: List[torch.Tensor] = []
         self.ops: List[NORMAL_T_T1_OP] = []
-        self.b = torch.empty((self.F,) + self.n, dtype=torch.complex64, device=dev)
-        img = torch.empty((self.C,) + self.n, dtype=torch.complex64, device=dev)
+        self.adj: List[hf.nufft_plan] = []
+        self.coords: List[List[torch.Tensor]] = []
+        self.kdata: List[torch.Tensor] = []
         for f in range(self.F):
             kf = [c.contiguous() for c in coords[f]]
+            y = kdata[f].contiguous()
+            if tuple(y.shape) != (self.C, int(kf[0].shape[0])):
+                raise RuntimeError(f"kdata[{f}] must have shape [{self.C}, {int(kf[0].shape[0])}]")
             kernel = torch.empty(tuple(2 * s for s in self.n), dtype=torch.complex64, device=dev)
             hf.toeplitz_kernel(kf, kernel, tol=kernel_tol)
             self.kernels.append(kernel)
             self.ops.append(NORMAL_T_T1_OP(kernel, self.smaps))
-            adj.setpts(kf)
-            y = kdata[f].contiguous()
-            if tuple(y.shape) != (self.C, int(kf[0].shape[0])):
-                raise RuntimeError(f"kdata[{f}] must have shape [{self.C}, {int(kf[0].shape[0])}]")
-            adj.execute(y, img)
-            torch.sum(self.smaps.conj() * img, dim=0, out=self.b[f])
-            self.b[f] /= float(self.nvox)
-        adj.free()
-        # CG workspace of all frames
+            plan = hf.nufft_plan.make(hf.nufft_opts(nmodes=self.n, ntransf=self.C, tol=tol,
+                                                    sign=hf.nufft_sign.DEFAULT_TYPE_1, **opts),
+                                      hf.nufft_type.TYPE_1)
+            plan.setpts(kf)
+            self.adj.append(plan)
+            self.coords.append(kf)
+            self.kdata.append(y)
+        self.nstreams = min(int(streams), self.F)
+        self.streams = [torch.cuda.Stream(device=dev) for _ in range(self.nstreams)]
+        # per frame: right-hand side and CG workspace
         shape = (self.F,) + self.n
+        self.b = torch.empty(shape, dtype=torch.complex64, device=dev)
         self.x = torch.zeros(shape, dtype=torch.complex64, device=dev)
         self.r = torch.empty_like(self.x)
         self.p = torch.empty_like(self.x)
         self.Ap = torch.empty_like(self.x)
         self.state = torch.zeros((self.F, CG_STATE_DOUBLES), dtype=torch.float64, device=dev)
-        self.graph: Optional[torch.cuda.CUDAGraph] = None
-
-    def _stream(self):
-        return torch.cuda.current_stream(self.device).cuda_stream
-
-    def _init(self):
-        self.x.zero_()
-        self.r.copy_(self.b)
-        s = self._stream()
-        n = self.nvox
+        self.graphs: Dict[tuple, torch.cuda.CUDAGraph] = {}
+        # the right-hand sides once, eagerly (they also warm every plan-owned buffer)
         for f in range(self.F):
-            check(lib().hasty_cg_init(self.r[f].data_ptr(), self.p[f].data_ptr(), n, 0.0,
-                                      self.state[f].data_ptr(), s))
+            self._rhs(f)
+        torch.cuda.synchronize(dev)
 
-    def _iteration(self):
-        """One CG iteration of every frame (kernel launches only: capturable)."""
-        s = self._stream()
-        n = self.nvox
-        for f in range(self.F):
-            self.ops[f].apply(self.p[f], self.Ap[f])
-            check(lib().hasty_cg_step(self.x[f].data_ptr(), self.r[f].data_ptr(), self.p[f].data_ptr(),
-                                      self.Ap[f].data_ptr(), n, self.state[f].data_ptr(), s))
+    # -- launches of one frame (capturable: kernels only, no allocation in the library) --
+    def _rhs(self, f: int) -> None:
+        """b_f = (1/N^d) sum_c conj(S_c) A_f^H y_f: one type-1 NUFFT of all
+        coils with conj(S_c) and the coil sum in its last FFT pass."""
+        plan = self.adj[f]
+        s = torch.cuda.current_stream(self.device).cuda_stream
+        check(lib().hasty_nufft_set_stream(plan._handle(), s))
+        check(lib().hasty_sense_adjoint(plan._handle(), self.C, 1, self.kdata[f].data_ptr(),
+                                        self.smaps.data_ptr(), None, None, self.b[f].data_ptr(), 1))
+        self.b[f].mul_(1.0 / float(self.nvox))
 
-    def solve(self, iters: int, graph: bool = True) -> torch.Tensor:
-        """``iters`` CG iterations of every frame from x = 0; returns x [F, N...]."""
-        self._init()
-        if iters <= 0:
-            return self.x
-        if not graph:
+    def _init(self, f: int) -> None:
+        s = torch.cuda.current_stream(self.device).cuda_stream
+        self.x[f].zero_()
+        torch.mul(self.b[f], 1.0, out=self.r[f])
+        check(lib().hasty_cg_init(self.r[f].data_ptr(), self.p[f].data_ptr(), self.nvox, 0.0,
+                                  self.state[f].data_ptr(), s))
+
+    def _iteration(self, f: int) -> None:
+        s = torch.cuda.current_stream(self.device).cuda_stream
+        self.ops[f].apply(self.p[f], self.Ap[f])
+        check(lib().hasty_cg_step(self.x[f].data_ptr(), self.r[f].data_ptr(), self.p[f].data_ptr(),
+                                  self.Ap[f].data_ptr(), self.nvox, self.state[f].data_ptr(), s))
+
+    def _branch(self, i: int, iters: int, rhs: bool) -> None:
+        """Frames i, i + streams, ...: right-hand side, CG init, iterations."""
+        for f in range(i, self.F, self.nstreams):
+            if rhs:
+                self._rhs(f)
+            self._init(f)
             for _ in range(iters):
-                self._iteration()
+                self._iteration(f)
+
+    def _solve_launches(self, iters: int, branches: bool, rhs: bool) -> None:
+        if not branches:
+            for f in range(self.F):
+                if rhs:
+                    self._rhs(f)
+                self._init(f)
+                for _ in range(iters):
+                    self._iteration(f)
+            return
+        main = torch.cuda.current_stream(self.device)
+        for i, st in enumerate(self.streams):
+            st.wait_stream(main)
+            with torch.cuda.stream(st):
+                self._branch(i, iters, rhs)
+        for st in self.streams:
+            main.wait_stream(st)
+
+    def solve(self, iters: int, graph: bool = True, branches: bool = True, rhs: bool = True) -> torch.Tensor:
+        """``iters`` CG iterations of every frame from x = 0; returns x [F, N...].
+
+        ``rhs``: recompute the right-hand sides from the k-space data first
+        (part of the solve; False reuses those of the last solve).
+        ``graph``: replay the captured solve (captured on first use for this
+        ``iters``; the capture itself executes nothing).  ``branches``: run the
+        frames on the concurrent streams (eager path; a graph always does).
+        """
+        if iters < 0:
+            raise RuntimeError("iters must be >= 0")
+        if not graph:
+            self._solve_launches(iters, branches, rhs)
             return self.x
-        # the first iteration runs eagerly (it also warms every plan-owned
-        # cache), the rest replay one captured iteration
-        self._iteration()
-        if self.graph is None and iters > 1:
+        key = (int(iters), bool(rhs))
+        g = self.graphs.get(key)
+        if g is None:
             torch.cuda.synchronize(self.device)
             g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g):
-                self._iteration()
-            self.graph = g
-            # capture records without executing: the state is as after one iteration
-        for _ in range(iters - 1):
-            self.graph.replay()
+                self._solve_launches(iters, True, rhs)
+            self.graphs[key] = g
+        g.replay()
         return self.x
 
     def residuals(self) -> torch.Tensor:

@@ -1,9 +1,11 @@
 """GPU: batched dynamic frames (hastycucompute_amd.frames) and the NUFFT
 autotuner (hastycucompute_amd.autotune).
 
-* The hipGraph replay of one CG iteration of every frame gives the same
-  result as the eager launches (bitwise: every kernel is deterministic), on
-  two solves in a row with eager work between them.
+* The hipGraph replay of the whole solve (right-hand sides and CG of every
+  frame, as concurrent branches) gives the same result as the eager launches
+  (bitwise: every kernel is deterministic), on three replays with eager work
+  between them; at full size (20 frames of 96^3 x 16 coils) too, with each
+  frame's normal operator Hermitian and positive semi-definite.
 * Each frame's CG iterate matches the oracle's FP64 CG on the oracle's
   Toeplitz normal operator (same iteration count) to 1e-4.
 * A tuned plan computes the same transform as the default plan.
@@ -37,19 +39,25 @@ def _frames_problem(n=(16, 16, 16), C=3, spokes=(40, 70, 100), samples=24):
 
 
 def test_frames_graph_equals_eager_and_oracle_cg():
+    """The solve (right-hand sides + CG) replayed from one graph of concurrent
+    branches equals the eager launches bitwise -- one frame after another on
+    one stream, and on the branches -- over three replays with eager work
+    between them, and every frame matches the oracle's FP64 CG."""
     from hastycucompute_amd.frames import FrameBatch
     n, S, coords, kdata = _frames_problem()
-    fb = FrameBatch(cuda(S), [[cuda(c) for c in k] for k in coords], [cuda(y) for y in kdata], tol=1e-6)
+    fb = FrameBatch(cuda(S), [[cuda(c) for c in k] for k in coords], [cuda(y) for y in kdata], tol=1e-6,
+                    streams=2)
     iters = 8
-    eager = fb.solve(iters, graph=False).clone()
-    g1 = fb.solve(iters, graph=True).clone()
-    torch.cuda.synchronize()
-    _ = (eager * 2).sum().item()            # eager work between the replays
-    g2 = fb.solve(iters, graph=True).clone()
-    torch.cuda.synchronize()
-    assert torch.equal(g1, eager), "graph replay differs from the eager launches"
-    assert torch.equal(g2, eager), "second graph solve differs from the eager launches"
-    print(f"[parity] frames: graph == eager bitwise over {iters} iterations x {len(coords)} frames", flush=True)
+    eager = fb.solve(iters, graph=False, branches=False).clone()
+    eager_br = fb.solve(iters, graph=False, branches=True).clone()
+    assert torch.equal(eager_br, eager), "eager branches differ from one stream"
+    for rep in range(3):
+        g = fb.solve(iters, graph=True).clone()
+        torch.cuda.synchronize()
+        _ = (eager * 2).sum().item()            # eager work between the replays
+        assert torch.equal(g, eager), f"graph replay {rep} differs from the eager launches"
+    print(f"[parity] frames: graph == eager bitwise over {iters} iterations x {len(coords)} frames, 3 replays",
+          flush=True)
     V = float(np.prod(n))
     for f in range(len(coords)):
         k64 = [c.astype(np.float64) for c in coords[f]]
@@ -58,6 +66,51 @@ def test_frames_graph_equals_eager_and_oracle_cg():
                 for c in range(S.shape[0])) / V
         xo = conjugate_gradient(lambda v: sense_normal(v, S, K), b, np.zeros(n, np.complex128), iters)
         report(f"frame {f} ({len(coords[f][0])} pts) CG vs oracle FP64 CG", rel_l2(host(eager[f]), xo), 1e-4)
+
+
+def test_frames_full_size_96c16x20():
+    """The dynamic workload at full size: 20 frames of 96^3 x 16 coils, spoke
+    counts U[500, 3000] x 192 samples.  The replayed solve equals the eager
+    one-frame-at-a-time launches bitwise; each frame's normal operator is
+    Hermitian and positive semi-definite (frames 0, 9, 19); the graph of
+    concurrent branches is timed against the eager launches (printed)."""
+    import time
+    from hastycucompute_amd import synth
+    from hastycucompute_amd.frames import FrameBatch
+    dev = torch.device("cuda", 0)
+    N, C, F, samples, iters = 96, 16, 20, 192, 15
+    rng = np.random.default_rng(3)
+    spokes = rng.integers(500, 3001, F)
+    n = (N, N, N)
+    S = torch.from_numpy(synth.coil_maps(n, C, seed=1)).to(dev)
+    g = torch.Generator(device=dev).manual_seed(5)
+    coords, kdata = [], []
+    for f, ns in enumerate(spokes):
+        coords.append([torch.from_numpy(c).to(dev) for c in synth.rotated_kooshball(int(ns), samples, f)])
+        kdata.append(torch.randn((C, int(ns) * samples), dtype=torch.complex64, device=dev, generator=g))
+    fb = FrameBatch(S, coords, kdata, tol=1e-5)
+    eager = fb.solve(iters, graph=False, branches=False).clone()
+    times = {}
+    for name, kw in (("eager", dict(graph=False, branches=False)), ("graph", dict(graph=True))):
+        fb.solve(iters, **kw)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x = fb.solve(iters, **kw)
+        torch.cuda.synchronize()
+        times[name] = time.perf_counter() - t0
+        assert torch.equal(x, eager), f"{name} solve differs from the eager launches"
+    print(f"[workload] frames 20 x 96^3 x 16, {iters} CG iterations + right-hand sides: eager "
+          f"{times['eager'] * 1e3:.1f} ms, graph of {fb.nstreams} branches {times['graph'] * 1e3:.1f} ms "
+          f"({times['eager'] / times['graph']:.2f}x), bitwise equal", flush=True)
+    for f in (0, 9, 19):
+        u = torch.randn(n, dtype=torch.complex64, device=dev, generator=g)
+        v = torch.randn(n, dtype=torch.complex64, device=dev, generator=g)
+        Nu, Nv = fb.ops[f](u), fb.ops[f](v)
+        a = complex(torch.vdot(u.reshape(-1).to(torch.complex128), Nv.reshape(-1).to(torch.complex128)))
+        b = complex(torch.vdot(Nu.reshape(-1).to(torch.complex128), v.reshape(-1).to(torch.complex128)))
+        report(f"frame {f} normal operator Hermitian <u, N v> vs <N u, v>", abs(a - b) / abs(a), 1e-5)
+        q = complex(torch.vdot(v.reshape(-1).to(torch.complex128), Nv.reshape(-1).to(torch.complex128)))
+        assert q.real > 0 and abs(q.imag) <= 1e-5 * q.real, (f, q)
 
 
 def test_autotune_choice_computes_same_transform():
