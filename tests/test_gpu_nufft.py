@@ -167,18 +167,21 @@ def test_fp64_plan():
     report("fp64 type2 tol 1e-10", rel_l2(host(o2)[0], ndft_type2(f[0], coords)), 1e-9)
 
 
-@pytest.mark.parametrize("kw", [{}, {"max_subproblem": 200}])
-def test_paired_interpolation_is_bitwise_single(kw):
+@pytest.mark.parametrize("spokes,kw", [(300, {}), (300, {"max_subproblem": 200}),
+                                       (2400, {"max_subproblem": 3000})])
+def test_paired_interpolation_is_bitwise_single(spokes, kw):
     """Type 2 on the 16^3 tile interpolates two transforms per block from
     interleaved LDS cells, after setpts reorders each subproblem's points by
     LDS bank class.  Each point's value is the same sum in the same order as
     in the one-transform kernel (an explicit coil_group keeps that kernel and
     the cell order), so the outputs agree bit for bit; odd batch (the last
-    pair stages a zero partner), and split subproblems with a small cap."""
+    pair stages a zero partner), split subproblems with a small cap, and
+    subproblems of several rounds of 1024 points with a partial last one
+    (2400 spokes: thousands of points per central tile, cap 3000)."""
     from hastycucompute_amd import fft as hf
     from hastycucompute_amd import synth
     shape, T = (24, 24, 24), 5
-    coords = synth.kooshball(300, 48)
+    coords = synth.kooshball(spokes, 48)
     M = len(coords[0])
     rng = np.random.default_rng(31)
     f = crandn(rng, (T,) + shape)
