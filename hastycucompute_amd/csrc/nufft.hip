@@ -592,7 +592,9 @@ NufftPlan* nufft_makeplan(int type, int dim, const int64_t* nmodes, int sign, in
         static const char* env = std::getenv("HASTY_INTERP_PAIR");
         const bool std3 = dim == 3 && p->T[0] == 16 && p->T[1] == 16 && p->T[2] == 16;
         void* fn = std3 ? nufft_interp_pair_ptr(dim, p->w) : nullptr;
-        if (fn && interp_lds_bytes(p.get(), 2) <= kLdsMax && !(env && env[0] == '0')) {
+        // (the kernel addresses one transform's fine grid with 32-bit byte offsets)
+        const bool small_grid = p->nf[0] * p->nf[1] * p->nf[2] * 8 <= int64_t(UINT32_MAX);
+        if (fn && small_grid && interp_lds_bytes(p.get(), 2) <= kLdsMax && !(env && env[0] == '0')) {
             p->pair = true;
             set_lds_limit(fn, interp_lds_bytes(p.get(), 2));
         }

@@ -215,31 +215,41 @@ __global__ void __launch_bounds__(kInterpPairThreads) interp_pair_kernel(GridArg
     const float2* f0 = a.fine + int64_t(g0) * G;
     const float2* f1 = two ? f0 + G : f0;
     // stage: every load of the tile in flight at once (one memory latency per
-    // block, which has the CU to itself while it stages)
+    // block, which has the CU to itself while it stages).  Thread t stages
+    // padded cells t + 1024 u: its first cell's (pz, py, px) once, then the
+    // compile-time steps of 1024 cells with their carries; the fine grid
+    // wraps with one unsigned min per axis (t0 + p < 2 nf) and the offsets
+    // are 32-bit byte offsets (the plan pairs transforms only when one
+    // transform's fine grid is under 4 GiB)
     constexpr int SU = (cells + kInterpPairThreads - 1) / kInterpPairThreads;
     {
         const int base = int(threadIdx.x);
+        const unsigned nf0 = unsigned(a.g.nf[0]), nf1 = unsigned(a.g.nf[1]), nf2 = unsigned(a.g.nf[2]);
+        const int px0 = base % Px, py0 = (base / Px) % Py, pz0 = base / (Px * Py);
         float2 v0[SU], v1[SU];
-        int off[SU];
 #pragma unroll
         for (int u = 0; u < SU; ++u) {
-            const int c = base + u * kInterpPairThreads;
-            off[u] = -1;
-            if (c < cells) {
-                const int px = c % Px;
-                const int py = (c / Px) % Py;
-                const int pz = c / (Px * Py);
-                int gz = t0[0] + pz; gz -= gz >= a.g.nf[0] ? int(a.g.nf[0]) : 0;
-                int gy = t0[1] + py; gy -= gy >= a.g.nf[1] ? int(a.g.nf[1]) : 0;
-                int gx = t0[2] + px; gx -= gx >= a.g.nf[2] ? int(a.g.nf[2]) : 0;
-                off[u] = int((int64_t(gz) * a.g.nf[1] + gy) * a.g.nf[2] + gx);
-                v0[u] = f0[off[u]];
-                v1[u] = two ? f1[off[u]] : make_float2(0.f, 0.f);
+            constexpr int kStep = kInterpPairThreads;
+            const int dx = (u * kStep) % Px, dy = ((u * kStep) / Px) % Py, dz = (u * kStep) / (Px * Py);
+            int px = px0 + dx, py = py0 + dy, pz = pz0 + dz;
+            if (px >= Px) { px -= Px; ++py; }
+            if (py >= Py) { py -= Py; ++pz; }
+            if (base + u * kStep < cells) {
+                unsigned gz = unsigned(t0[0] + pz), gy = unsigned(t0[1] + py), gx = unsigned(t0[2] + px);
+                gz = min(gz, gz - nf0);
+                gy = min(gy, gy - nf1);
+                gx = min(gx, gx - nf2);
+                // byte offset from the (uniform) grid base: a 32-bit VGPR offset
+                const unsigned boff = ((gz * nf1 + gy) * nf2 + gx) * unsigned(sizeof(float2));
+                v0[u] = *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(f0) + boff);
+                v1[u] = *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(f1) + boff);
+                if (!two) v1[u] = make_float2(0.f, 0.f);
             }
         }
 #pragma unroll
         for (int u = 0; u < SU; ++u)
-            if (off[u] >= 0) lds[base + u * kInterpPairThreads] = make_float4(v0[u].x, v0[u].y, v1[u].x, v1[u].y);
+            if (base + u * kInterpPairThreads < cells)
+                lds[base + u * kInterpPairThreads] = make_float4(v0[u].x, v0[u].y, v1[u].x, v1[u].y);
     }
     __syncthreads();
     for (int s = sub.y + threadIdx.x; s < sub.y + sub.z; s += blockDim.x) {
