@@ -183,7 +183,7 @@ PMC_KERNELS = {
     "toep_P3_z_psf": "fft2_pass_kernel<HIP_vector_type<float, 2u>, 16, 16, -1, 1, 3>",
     "toep_F45_yx_ifft_crop_sum": "toep_crop_sum_kernel",
     "toep_F12_xy_pad_fft": "toep_pad_fft_kernel",
-    "nufft_spread": "spread_kernel_w6<HIP_vector_type<float, 2u>, 32>",
+    "nufft_spread": "spread_lanes_kernel<6, 32>",
     "nufft_interp": "interp_pair_kernel<6, 16, 16, 16>",
 }
 CALIB_SUBSTR = "neg_kernel"
@@ -250,7 +250,7 @@ _TIMED_SEQ = ([("toep_F12_xy_pad_fft", "toep_pad_fft_kernel"),
                ("toep_F45 sum_partials", "sum_partials")] +
               [("fwd_xy_pad_fft", "nufft_fwd_xy_kernel"),
                ("fwd_z_pad_fft", "<c64, 16, 16, -1, 0, 1>"), ("nufft_interp", "interp_pair_kernel<6,")] +
-              [("nufft_sort_values", "sort_values_kernel<c64"), ("nufft_spread", "spread_kernel_w6<c64, 32>"),
+              [("nufft_sort_values", "sort_values_kernel<c64"), ("nufft_spread", "spread_lanes_kernel<6, 32>"),
                ("nufft_spread_reduce", "reduce_split_tiles"), ("inv_z_fft_crop", "<c64, 16, 16, 1, 0, 4>"),
                ("inv_y_fft_crop", "<c64, 16, 16, 1, 0, 4>"), ("inv_x_fft_crop", "<c64, 16, 16, 1, 3, 4>")]) * 3
 
@@ -318,9 +318,9 @@ def summarize_trace(trace_dir, workload, top=30):
 
 _SQ_PASSES = ("SQ_INSTS_VALU SQ_INSTS_SALU SQ_INSTS_SMEM SQ_INSTS_VMEM_RD SQ_WAIT_INST_ANY SQ_WAVE_CYCLES "
               "SQ_BUSY_CYCLES SQ_ACTIVE_INST_VALU",
-              "SQ_WAVES SQ_INSTS_VALU_FMA_F32 SQ_INSTS_LDS SQ_LDS_BANK_CONFLICT SQ_ACTIVE_INST_LDS GRBM_GUI_ACTIVE")
-_SQ_KERNELS = (("spread_kernel_w6<c64, 32>", "spread_kernel_w6<HIP_vector_type<float, 2u>, 32>", 20_000_000),
-               ("spread_lanes_kernel<6, 32>", "spread_lanes_kernel<6, 32>", 10_000_000),
+              "SQ_WAVES SQ_INSTS_VALU_FMA_F32 SQ_INSTS_LDS SQ_LDS_BANK_CONFLICT SQ_ACTIVE_INST_LDS GRBM_GUI_ACTIVE",
+              "SQ_INSTS_MFMA SQ_VALU_MFMA_BUSY_CYCLES SQ_INSTS_VMEM_WR SQ_WAIT_ANY")
+_SQ_KERNELS = (("spread_lanes_kernel<6, 32>", "spread_lanes_kernel<6, 32>", 10_000_000),
                ("interp_pair_kernel<6, 16, 16, 16>", "interp_pair_kernel<6, 16, 16, 16>", 40_000_000))
 
 
@@ -334,17 +334,19 @@ _SQ_TOEP_KERNELS = (("toep_F12_xy_pad_fft", "toep_pad_fft_kernel", 100_000),
                     ("toep_F45_yx_ifft_crop_sum", "toep_crop_sum_kernel", 100_000))
 
 
-def summarize_sq(dir_a, dir_b, workload, toeplitz=False):
-    """Markdown summary of the two SQ-counter passes over the NUFFT spread and
-    interpolation kernels, or (toeplitz) over the passes of the A^H A apply
-    (full-size launches only)."""
+def summarize_sq(dirs, workload, toeplitz=False):
+    """Markdown summary of the SQ-counter passes (one directory per pass)
+    over the NUFFT spread and interpolation kernels, or (toeplitz) over the
+    passes of the A^H A apply (full-size launches only)."""
     kernels = _SQ_TOEP_KERNELS if toeplitz else _SQ_KERNELS
     passes = _SQ_TOEP_PASSES if toeplitz else _SQ_PASSES
+    if len(dirs) != len(passes):
+        raise SystemExit(f"expected {len(passes)} pass directories, got {len(dirs)}")
     import collections
     import csv
     import glob
     res = collections.defaultdict(dict)
-    for d in (dir_a, dir_b):
+    for d in dirs:
         agg = collections.defaultdict(lambda: collections.defaultdict(list))
         for f in glob.glob(os.path.join(d, "**", "*counter_collection*.csv"), recursive=True):
             for r in csv.DictReader(open(f)):
@@ -361,9 +363,9 @@ def summarize_sq(dir_a, dir_b, workload, toeplitz=False):
     title = "Toeplitz passes of the A^H A apply" if toeplitz else "NUFFT spread and interpolation"
     flag = "--summarize-sq-toeplitz" if toeplitz else "--summarize-sq"
     out = [f"# SQ counters: {title}, {workload}", "",
-           "Two counter passes (the hardware takes at most 8 SQ counters per pass), from the repository root on one "
-           "MI355X, then the summary:", "", "```", cmd.format(c=passes[0], d=dir_a), cmd.format(c=passes[1], d=dir_b),
-           f"python bench.py {flag} {dir_a} {dir_b}", "```", "",
+           f"{len(passes)} counter passes (the hardware takes at most 8 SQ counters per pass), from the repository "
+           "root on one MI355X, then the summary:", "", "```"] + [cmd.format(c=c, d=d) for c, d in zip(passes, dirs)] + [
+           f"python bench.py {flag} {' '.join(dirs)}", "```", "",
            "Values are per launch (mean over the full-size launches), in units of 10^9. Instruction counts are "
            "wave-instructions summed over the GPU; `GRBM_GUI_ACTIVE` is summed over the 8 XCDs.", "",
            "| counter | " + " | ".join(f"`{k}`" for k in labels) + " |", "|---|" + "---:|" * len(labels)]
@@ -372,6 +374,17 @@ def summarize_sq(dir_a, dir_b, workload, toeplitz=False):
     out.append("")
     for k in labels:
         v, f = res[k].get("SQ_INSTS_VALU"), res[k].get("SQ_INSTS_VALU_FMA_F32")
+        mf, act = res[k].get("SQ_INSTS_MFMA"), res[k].get("GRBM_GUI_ACTIVE")
+        if mf and act:
+            # v_mfma_f32_32x32x2_f32: 32 x 32 x 2 multiply-adds per wave-instruction
+            busy = res[k].get("SQ_VALU_MFMA_BUSY_CYCLES")
+            line = (f"- `{k}`: {mf / 1e6:.0f} M MFMA wave-instructions = {mf * 4096 / 1e12:.2f} TFLOP issued "
+                    f"per launch (zero weights included), over {act / 8 / 1e6:.2f} M GPU clock cycles "
+                    f"(GRBM_GUI_ACTIVE per XCD)")
+            if busy:
+                # summed over the 1024 SIMDs (256 CUs x 4)
+                line += f"; MFMA busy {100 * busy / (1024 * act / 8):.0f}% of the SIMD cycles"
+            out.append(line + ".")
         if v and f:
             out.append(f"- `{k}`: VALU-issue floor {v / 256 / 2.4e9 * 1e3:.1f} ms (one wave-instruction per CU per "
                        f"cycle, 256 CUs, 2.4 GHz); FP32 FMAs are {100 * f / v:.0f}% of its VALU instructions.")
@@ -483,8 +496,8 @@ def main():
     ap.add_argument("--summarize-trace", metavar="TRACE_DIR",
                     help="summarise a rocprofv3 --kernel-trace run of this script into "
                          "profiles/<workload>_kernel_stats.md and exit")
-    ap.add_argument("--summarize-sq", nargs=2, metavar=("PASS1_DIR", "PASS2_DIR"),
-                    help="summarise the two SQ-counter passes over the NUFFT kernels into "
+    ap.add_argument("--summarize-sq", nargs=3, metavar=("PASS1_DIR", "PASS2_DIR", "PASS3_DIR"),
+                    help="summarise the three SQ-counter passes over the NUFFT kernels into "
                          "profiles/nufft_spread_interp_sq_counters_<workload>.md and exit")
     ap.add_argument("--summarize-sq-toeplitz", nargs=2, metavar=("PASS1_DIR", "PASS2_DIR"),
                     help="summarise the two SQ-counter passes over the Toeplitz passes into "
@@ -497,10 +510,10 @@ def main():
         summarize_trace(args.summarize_trace, args.workload)
         return
     if args.summarize_sq:
-        summarize_sq(args.summarize_sq[0], args.summarize_sq[1], args.workload)
+        summarize_sq(args.summarize_sq, args.workload)
         return
     if args.summarize_sq_toeplitz:
-        summarize_sq(args.summarize_sq_toeplitz[0], args.summarize_sq_toeplitz[1], args.workload, toeplitz=True)
+        summarize_sq(args.summarize_sq_toeplitz, args.workload, toeplitz=True)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "

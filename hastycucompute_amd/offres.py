@@ -57,8 +57,7 @@ def histogram(fieldmap: torch.Tensor, nbins: int, b0_only: bool = False,
     if b0_only:
         v = z.imag
         vmin, vmax = float(v.min()), float(v.max())
-        h = torch.zeros(nbins, dtype=torch.float64, device=dev)
-        h.index_add_(0, _bin_index(v, vmin, vmax, nbins), torch.ones_like(v))
+        h = torch.bincount(_bin_index(v, vmin, vmax, nbins), minlength=nbins).to(torch.float64)
         c = _centres(vmin, vmax, nbins, dev)
         if autocorrelate:
             # counts of differences between bin centres: full correlation
@@ -72,8 +71,8 @@ def histogram(fieldmap: torch.Tensor, nbins: int, b0_only: bool = False,
     rmin, rmax, imin, imax = float(re.min()), float(re.max()), float(im.min()), float(im.max())
     ir = _bin_index(re, rmin, rmax, nbins)
     ii = _bin_index(im, imin, imax, nbins)
-    h = torch.zeros(nbins * nbins, dtype=torch.float64, device=dev)
-    h.index_add_(0, ir * nbins + ii, torch.ones_like(re))
+    # integer counts (an FP64 index_add_ of ones contends on a few bins)
+    h = torch.bincount(ir * nbins + ii, minlength=nbins * nbins).to(torch.float64)
     cr = _centres(rmin, rmax, nbins, dev)
     ci = _centres(imin, imax, nbins, dev)
     zk = torch.complex(cr.view(-1, 1).expand(nbins, nbins), ci.view(1, -1).expand(nbins, nbins)).reshape(-1)
