@@ -412,8 +412,10 @@ def measured_traffic(workload, tag):
 def bench_frames(dev, iters=15, nframes=20, N=96, C=16, samples=192, seed=3, reps=3):
     """Batched dynamic frames: ``nframes`` rotated golden-means kooshballs of
     U[500, 3000] spokes x ``samples`` at N^3 with C coils, 15 CG iterations
-    per frame on each frame's Toeplitz normal operator; eager launches vs one
-    captured hipGraph iteration replayed.  Returns the timings."""
+    per frame on each frame's Toeplitz normal operator.  Timed: eager
+    per-frame launches (one stream, then concurrent branches), the per-frame
+    solve replayed from one hipGraph, and the batched solve (one launch per
+    pass over all frames), eager and from a graph.  Returns the timings."""
     import numpy as np
     import torch
 
@@ -447,8 +449,11 @@ def bench_frames(dev, iters=15, nframes=20, N=96, C=16, samples=192, seed=3, rep
     # one solve = every frame's right-hand side (type-1 NUFFT of its data) and
     # `iters` CG iterations; eager one frame after another on one stream, eager
     # on the concurrent branches, and the whole solve replayed from one graph
-    modes = {"eager": dict(graph=False, branches=False), "eager_branches": dict(graph=False, branches=True),
-             "graph": dict(graph=True)}
+    modes = {"eager": dict(graph=False, branches=False, batched=False),
+             "eager_branches": dict(graph=False, branches=True, batched=False),
+             "graph": dict(graph=True, batched=False),
+             "batched_eager": dict(graph=False, batched=True),
+             "batched_graph": dict(graph=True, batched=True)}
     out = {}
     for mode, kw in modes.items():
         fb.solve(iters, **kw)       # warm (and capture)
@@ -458,13 +463,16 @@ def bench_frames(dev, iters=15, nframes=20, N=96, C=16, samples=192, seed=3, rep
             fb.solve(iters, **kw)
         torch.cuda.synchronize()
         out[mode] = (time.perf_counter() - t) / reps
-    xe = fb.solve(iters, graph=False, branches=False).clone()
-    xg = fb.solve(iters, graph=True).clone()
-    same = bool(torch.equal(xe, xg))
+    xe = fb.solve(iters, graph=False, branches=False, batched=False).clone()
+    xg = fb.solve(iters, graph=True, batched=False).clone()
+    xb = fb.solve(iters, graph=True, batched=True).clone()
+    same = bool(torch.equal(xe, xg)) and bool(torch.equal(xe, xb))
+    best = min(out["graph"], out["batched_graph"])
     total_pts = int(sum(int(c[0].shape[0]) for c in coords))
     log(f"[bench] frames: {nframes} x {N}^3 x {C} coils, {total_pts} pts, {iters} CG iters: "
         f"eager {out['eager'] * 1e3:.1f} ms, eager branches {out['eager_branches'] * 1e3:.1f} ms, "
-        f"graph {out['graph'] * 1e3:.1f} ms, identical={same}")
+        f"graph {out['graph'] * 1e3:.1f} ms, batched eager {out['batched_eager'] * 1e3:.1f} ms, "
+        f"batched graph {out['batched_graph'] * 1e3:.1f} ms, identical={same}")
     return {"frames": nframes, "image": [N] * 3, "coils": C, "cg_iters": iters,
             "spokes_per_frame": [int(s) for s in spokes], "samples_per_spoke": samples,
             "total_points_per_coil": total_pts,
@@ -473,8 +481,11 @@ def bench_frames(dev, iters=15, nframes=20, N=96, C=16, samples=192, seed=3, rep
             "eager_ms_per_solve": round(out["eager"] * 1e3, 3),
             "eager_branches_ms_per_solve": round(out["eager_branches"] * 1e3, 3),
             "graph_ms_per_solve": round(out["graph"] * 1e3, 3),
+            "batched_eager_ms_per_solve": round(out["batched_eager"] * 1e3, 3),
+            "batched_graph_ms_per_solve": round(out["batched_graph"] * 1e3, 3),
             "graph_speedup": round(out["eager"] / out["graph"], 3),
-            "frame_iterations_per_s": round(nframes * iters / out["graph"], 1),
+            "batched_graph_speedup": round(out["eager"] / out["batched_graph"], 3),
+            "frame_iterations_per_s": round(nframes * iters / best, 1),
             "graph_equals_eager": same,
             "setup_s": round(setup_s, 2), "autotune_s": round(tune_s, 2),
             "autotuned_type1_opts": autotune.plan_options(tuned)}
@@ -705,8 +716,9 @@ def main():
                 "traffic": round(traffic) if traffic else None,
                 "algorithmic_bytes_per_launch": ab[dom], "avg_launch_ms": round(toep[dom][0], 4)}
     # the NUFFT half of the metric: HBM view of interpolation and spreading,
-    # and the matrix-core view of spreading (8 C M w^3 useful flops: a
-    # complex accumulation per coil per window cell, run as FP32 MFMA)
+    # and the matrix-core view of spreading (4 C M w^3 useful flops: a
+    # complex value times a real weight, added, per coil per window cell --
+    # two FP32 multiply-adds -- run as FP32 MFMA)
     roof_nufft = {}
     w = p2.info()["width"]
     for tag, kname in (("type2", "nufft_interp"), ("type1", "nufft_spread")):
@@ -719,7 +731,7 @@ def main():
                  "traffic": round(traffic) if traffic else None, "algorithmic_bytes_per_launch": ab[kname],
                  "avg_launch_ms": round(kern_avg[kname][0], 4)}
             if kname == "nufft_spread":
-                useful = 8.0 * C * M * w ** 3
+                useful = 4.0 * C * M * w ** 3
                 e["useful_tflops"] = round(useful / t / 1e12, 2)
                 e["frac_of_fp32_mfma_157TF"] = round(useful / t / 157.3e12, 4)
             roof_nufft[tag] = e

@@ -69,6 +69,8 @@ SIGNATURES = {
                                         ctypes.POINTER(c_vp)]),
     "hasty_normal_t2_create": (c_int, [c_int, P_i64, c_int, c_vp, c_int, c_vp, c_vp, c_vp, c_int, c_vp,
                                        ctypes.POINTER(c_vp)]),
+    "hasty_normal_frames_create": (c_int, [c_int, P_i64, c_int, c_int, c_vp, c_vp, c_int, c_vp,
+                                           ctypes.POINTER(c_vp)]),
     "hasty_sense_forward": (c_int, [c_vp, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "hasty_sense_adjoint": (c_int, [c_vp, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_int]),
     "hasty_normal_apply": (c_int, [c_vp, c_vp, c_vp]),
@@ -78,6 +80,8 @@ SIGNATURES = {
     "hasty_dot_noalloc": (c_int, [c_vp, c_vp, c_i64, c_vp, c_vp]),
     "hasty_cg_init": (c_int, [c_vp, c_vp, c_i64, c_dbl, c_vp, c_vp]),
     "hasty_cg_step": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp]),
+    "hasty_cg_init_batched": (c_int, [c_vp, c_vp, c_i64, c_int, c_dbl, c_vp, c_vp]),
+    "hasty_cg_step_batched": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_vp, c_vp]),
     "hasty_cg_status": (c_int, [c_vp, P_dbl, P_int, c_vp]),
     "hasty_pcg_init": (c_int, [c_vp, c_vp, c_vp, c_i64, c_dbl, c_vp, c_vp]),
     "hasty_pcg_update": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp]),

@@ -36,6 +36,11 @@ __device__ inline double block_sum(double v, double* sh) {
 __global__ void __launch_bounds__(kRedThreads) dot_partial_kernel(const float2* a, const float2* b,
                                                                   int64_t n, double* part,
                                                                   const double* done) {
+    // batched systems (hasty_cg_*_batched): system blockIdx.y of [nsys, n]
+    // vectors and [nsys, HASTY_CG_STATE_DOUBLES] states
+    const int64_t sy = blockIdx.y;
+    a += sy * n; b += sy * n; part += sy * HASTY_CG_STATE_DOUBLES;
+    if (done) done += sy * HASTY_CG_STATE_DOUBLES;
     if (done && *done != 0.0) return;
     __shared__ double sh[kRedThreads];
     double re = 0.0, im = 0.0;
@@ -72,6 +77,7 @@ __global__ void __launch_bounds__(kRedThreads) sum_partials_kernel(const double*
 // CG: alpha = rz / pAp
 __global__ void cg_alpha_kernel(double* st) {
     __shared__ double sh[kRedThreads];
+    st += int64_t(blockIdx.y) * HASTY_CG_STATE_DOUBLES;
     if (st[ST_DONE] != 0.0) return;
     double v = 0.0;
     for (int i = threadIdx.x; i < kRedBlocks; i += blockDim.x) v += st[ST_PART + i];
@@ -92,6 +98,10 @@ __global__ void cg_alpha_kernel(double* st) {
 __global__ void __launch_bounds__(kRedThreads) cg_update_kernel(float2* x, float2* r, const float2* p,
                                                                 const float2* Ap, int64_t n, double* st,
                                                                 int want_norm) {
+    {
+        const int64_t sy = blockIdx.y;
+        x += sy * n; r += sy * n; p += sy * n; Ap += sy * n; st += sy * HASTY_CG_STATE_DOUBLES;
+    }
     if (st[ST_DONE] != 0.0) return;
     __shared__ double sh[kRedThreads];
     const double alpha = st[ST_ALPHA];
@@ -131,6 +141,7 @@ __global__ void pcg_beta_kernel(double* st, int restart) {
 // rz' = sum ; beta = rz'/rz ; rz = rz' ; done = sqrt(rz) < tol
 __global__ void cg_beta_kernel(double* st) {
     __shared__ double sh[kRedThreads];
+    st += int64_t(blockIdx.y) * HASTY_CG_STATE_DOUBLES;
     if (st[ST_DONE] != 0.0) return;
     double v = 0.0;
     for (int i = threadIdx.x; i < kRedBlocks; i += blockDim.x) v += st[ST_PART + kRedBlocks + i];
@@ -148,6 +159,7 @@ __global__ void cg_beta_kernel(double* st) {
 // p = r + beta p, then mark convergence
 __global__ void __launch_bounds__(kRedThreads) cg_p_kernel(float2* p, const float2* r, int64_t n,
                                                            const double* st) {
+    p += int64_t(blockIdx.y) * n; r += int64_t(blockIdx.y) * n; st += int64_t(blockIdx.y) * HASTY_CG_STATE_DOUBLES;
     if (st[ST_DONE] != 0.0) return;
     const float beta = float(st[ST_BETA]);
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
@@ -161,6 +173,7 @@ __global__ void __launch_bounds__(kRedThreads) cg_p_kernel(float2* p, const floa
 }
 
 __global__ void cg_finish_kernel(double* st) {
+    st += int64_t(blockIdx.y) * HASTY_CG_STATE_DOUBLES;
     if (st[ST_DONE] != 0.0) return;
     if (sqrt(st[ST_RZ]) < st[ST_TOL]) st[ST_DONE] = 1.0;
 }
@@ -168,6 +181,7 @@ __global__ void cg_finish_kernel(double* st) {
 __global__ void __launch_bounds__(kRedThreads) copy_norm_kernel(const float2* r, float2* p, int64_t n,
                                                                 double* st) {
     __shared__ double sh[kRedThreads];
+    r += int64_t(blockIdx.y) * n; p += int64_t(blockIdx.y) * n; st += int64_t(blockIdx.y) * HASTY_CG_STATE_DOUBLES;
     double acc = 0.0;
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
          i += int64_t(gridDim.x) * blockDim.x) {
@@ -196,6 +210,7 @@ __global__ void __launch_bounds__(kRedThreads) copy_dot_kernel(const float2* r, 
 
 __global__ void cg_init_finish_kernel(double* st, double tol) {
     __shared__ double sh[kRedThreads];
+    st += int64_t(blockIdx.y) * HASTY_CG_STATE_DOUBLES;
     double v = 0.0;
     for (int i = threadIdx.x; i < kRedBlocks; i += blockDim.x) v += st[ST_PART + kRedBlocks + i];
     v = block_sum(v, sh);
@@ -364,37 +379,63 @@ int hasty_axpy(double a_re, double a_im, const void* x, void* y, int64_t n, void
     });
 }
 
+static void cg_init_launch(void* r, void* p, int64_t n, int nsys, double tol, double* state, hipStream_t s) {
+    // unused partial slots must read as zero
+    hipLaunchKernelGGL(zero_doubles_kernel, dim3(4), dim3(256), 0, s, state, nsys * HASTY_CG_STATE_DOUBLES);
+    hipLaunchKernelGGL(copy_norm_kernel, dim3(red_blocks(n), unsigned(nsys)), dim3(kRedThreads), 0, s,
+                       static_cast<const float2*>(r), static_cast<float2*>(p), n, state);
+    hipLaunchKernelGGL(cg_init_finish_kernel, dim3(1, unsigned(nsys)), dim3(kRedThreads), 0, s, state, tol);
+    HASTY_CHECK_HIP(hipGetLastError());
+}
+
 int hasty_cg_init(void* r, void* p, int64_t n, double tol, double* state, void* stream) {
     return guarded([&] {
         HASTY_REQUIRE(r && p && state && n >= 0, "invalid arguments to hasty_cg_init");
-        hipStream_t s = static_cast<hipStream_t>(stream);
-        // unused partial slots must read as zero
-        hipLaunchKernelGGL(zero_doubles_kernel, dim3(4), dim3(256), 0, s, state, HASTY_CG_STATE_DOUBLES);
-        hipLaunchKernelGGL(copy_norm_kernel, dim3(red_blocks(n)), dim3(kRedThreads), 0, s,
-                           static_cast<const float2*>(r), static_cast<float2*>(p), n, state);
-        hipLaunchKernelGGL(cg_init_finish_kernel, dim3(1), dim3(kRedThreads), 0, s, state, tol);
-        HASTY_CHECK_HIP(hipGetLastError());
+        cg_init_launch(r, p, n, 1, tol, state, static_cast<hipStream_t>(stream));
     });
+}
+
+int hasty_cg_init_batched(void* r, void* p, int64_t n, int nsys, double tol, double* state, void* stream) {
+    return guarded([&] {
+        HASTY_REQUIRE(r && p && state && n >= 0, "invalid arguments to hasty_cg_init_batched");
+        HASTY_REQUIRE(nsys >= 1 && nsys <= 65535, "hasty_cg_init_batched: nsys must be in [1, 65535]");
+        cg_init_launch(r, p, n, nsys, tol, state, static_cast<hipStream_t>(stream));
+    });
+}
+
+// One CG update of nsys independent systems (blockIdx.y = system).
+static void cg_step_launch(void* x, void* r, void* p, const void* Ap, int64_t n, int nsys, double* state,
+                           hipStream_t s) {
+    const unsigned nb = red_blocks(n);
+    const unsigned ny = unsigned(nsys);
+    hipLaunchKernelGGL(dot_partial_kernel, dim3(nb, ny), dim3(kRedThreads), 0, s,
+                       static_cast<const float2*>(p), static_cast<const float2*>(Ap), n,
+                       state + ST_PART, static_cast<const double*>(state + ST_DONE));
+    hipLaunchKernelGGL(cg_alpha_kernel, dim3(1, ny), dim3(kRedThreads), 0, s, state);
+    hipLaunchKernelGGL(cg_update_kernel, dim3(nb, ny), dim3(kRedThreads), 0, s,
+                       static_cast<float2*>(x), static_cast<float2*>(r),
+                       static_cast<const float2*>(p), static_cast<const float2*>(Ap), n, state, 1);
+    hipLaunchKernelGGL(cg_beta_kernel, dim3(1, ny), dim3(kRedThreads), 0, s, state);
+    hipLaunchKernelGGL(cg_p_kernel, dim3(nb, ny), dim3(kRedThreads), 0, s, static_cast<float2*>(p),
+                       static_cast<const float2*>(r), n, static_cast<const double*>(state));
+    hipLaunchKernelGGL(cg_finish_kernel, dim3(1, ny), dim3(1), 0, s, state);
+    HASTY_CHECK_HIP(hipGetLastError());
 }
 
 int hasty_cg_step(void* x, void* r, void* p, const void* Ap, int64_t n, double* state,
                   void* stream) {
     return guarded([&] {
         HASTY_REQUIRE(x && r && p && Ap && state && n >= 0, "invalid arguments to hasty_cg_step");
-        hipStream_t s = static_cast<hipStream_t>(stream);
-        const unsigned nb = red_blocks(n);
-        hipLaunchKernelGGL(dot_partial_kernel, dim3(nb), dim3(kRedThreads), 0, s,
-                           static_cast<const float2*>(p), static_cast<const float2*>(Ap), n,
-                           state + ST_PART, static_cast<const double*>(state + ST_DONE));
-        hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(kRedThreads), 0, s, state);
-        hipLaunchKernelGGL(cg_update_kernel, dim3(nb), dim3(kRedThreads), 0, s,
-                           static_cast<float2*>(x), static_cast<float2*>(r),
-                           static_cast<const float2*>(p), static_cast<const float2*>(Ap), n, state, 1);
-        hipLaunchKernelGGL(cg_beta_kernel, dim3(1), dim3(kRedThreads), 0, s, state);
-        hipLaunchKernelGGL(cg_p_kernel, dim3(nb), dim3(kRedThreads), 0, s, static_cast<float2*>(p),
-                           static_cast<const float2*>(r), n, static_cast<const double*>(state));
-        hipLaunchKernelGGL(cg_finish_kernel, dim3(1), dim3(1), 0, s, state);
-        HASTY_CHECK_HIP(hipGetLastError());
+        cg_step_launch(x, r, p, Ap, n, 1, state, static_cast<hipStream_t>(stream));
+    });
+}
+
+int hasty_cg_step_batched(void* x, void* r, void* p, const void* Ap, int64_t n, int nsys, double* state,
+                          void* stream) {
+    return guarded([&] {
+        HASTY_REQUIRE(x && r && p && Ap && state && n >= 0, "invalid arguments to hasty_cg_step_batched");
+        HASTY_REQUIRE(nsys >= 1 && nsys <= 65535, "hasty_cg_step_batched: nsys must be in [1, 65535]");
+        cg_step_launch(x, r, p, Ap, n, nsys, state, static_cast<hipStream_t>(stream));
     });
 }
 

@@ -174,9 +174,24 @@ void launch_col_pass(ColArgs<V> a, hipStream_t s) {
         p.in_len = a.in_len; p.in_shift = a.in_shift; p.out_len = a.out_len; p.out_shift = a.out_shift;
         p.scale = a.scale;
         p.psf = a.psf; p.psf_o = a.psf_o; p.psf_i = 1; p.psf_s = a.psf_s;
+        p.psf_f = a.psf_f; p.psf_div = a.psf ? a.psf_div : 0;
         p.tw = a.tw;
         p.psf_q = a.psf_q;
         if (launch_fft2_pass<V>(p, a.L, a.sign, a.psf ? PM_COL_PSF : PM_COL, a.tag ? a.tag : "col_pass", s)) return;
+    }
+    if (a.psf && a.psf_div > 0 && a.psf_div < a.nbatch) {
+        // frames on the Stockham kernels: one launch per frame's kernel
+        HASTY_REQUIRE(a.nbatch % a.psf_div == 0, "col pass: batch is not a whole number of frames");
+        for (int f = 0; f < a.nbatch / a.psf_div; ++f) {
+            ColArgs<V> g = a;
+            g.in = a.in + int64_t(f) * a.psf_div * a.in_b;
+            g.out = a.out + int64_t(f) * a.psf_div * a.out_b;
+            g.psf = a.psf + int64_t(f) * a.psf_f;
+            g.nbatch = a.psf_div;
+            g.psf_div = 0;
+            launch_col_pass<V>(g, s);
+        }
+        return;
     }
     const int64_t blocks = ceil_div(a.ni, a.tx) * int64_t(a.no) * a.nbatch;
     if (blocks == 0) return;

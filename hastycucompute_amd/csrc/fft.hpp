@@ -187,6 +187,7 @@ template <class V> struct DiagOp {
     const V* ptr = nullptr;   // element (b, r, j) at ptr[b*sb + r*sr + j]
     int64_t sb = 0, sr = 0;
     int type = HASTY_MULT_NONE;
+    int64_t sf = 0;           // frame stride: passes batched over frames read frame f at ptr + f*sf
 };
 
 template <class V> struct RowLoadArgs {
@@ -231,6 +232,9 @@ template <class V> struct ColArgs {
     FFTPlan1D plan; const V* tw;
     const char* tag;
     int psf_q = 0;                           // class-major kernel columns: line i reads 4 (i % psf_q) + i / psf_q
+    // frames: batch element b belongs to frame b / psf_div, whose kernel is
+    // psf + (b / psf_div) * psf_f (psf_div = 0: one kernel for the batch)
+    int64_t psf_f = 0; int psf_div = 0;
 };
 
 constexpr int kRowAccMax = 16;

@@ -171,6 +171,7 @@ template <class V> struct PassArgs {
     int ny;                             // line i -> (z = i / ny, y = i % ny)
     int dz_on_load;                     // deapodisation applied on load (1) or store (0)
     const V* psf; int64_t psf_o, psf_i, psf_s;
+    int64_t psf_f; int psf_div;         // frames: element b uses the kernel at psf + (b / psf_div) * psf_f
     int bchunk;                         // PSF passes: batch elements per block (kernel read once)
     int accumulate, sum_batch;
     V* partial; int64_t partial_elems;  // scratch for a batch sum in block groups:
@@ -355,7 +356,9 @@ __global__ void __launch_bounds__(512) fft2_pass_kernel(PassArgs<V> a) {
     V kv[PSF ? L2 : 1];
     if constexpr (PSF) {
         if (t < L1) {
-            const char* kb = reinterpret_cast<const char*>(a.psf + int64_t(o) * a.psf_o);
+            // (a block's batch chunk lies within one frame: bchunk divides psf_div)
+            const int64_t kf = a.psf_div ? (b_first / a.psf_div) * a.psf_f : 0;
+            const char* kb = reinterpret_cast<const char*>(a.psf + kf + int64_t(o) * a.psf_o);
             const int kcol = a.psf_q ? 4 * (ic % a.psf_q) + ic / a.psf_q : ic;
             const uint32_t ko = uint32_t(kcol) * a.psf_ib + uint32_t(t) * a.psf_sb;
 #pragma unroll
@@ -684,8 +687,11 @@ __global__ void __launch_bounds__(512) fft2_pass_kernel(PassArgs<V> a) {
 template <class V>
 __global__ void __launch_bounds__(256) sum_partials_kernel(const V* __restrict__ partial, V* out, int64_t nrows,
                                                            int out_len, int64_t out_r, int groups,
-                                                           int accumulate) {
+                                                           int accumulate, int64_t out_f = 0) {
     const int64_t n = nrows * out_len;
+    // frames (gridDim.y): frame f's partials follow frame f-1's groups
+    partial += int64_t(blockIdx.y) * groups * n;
+    out += int64_t(blockIdx.y) * out_f;
     for (int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < n;
          idx += int64_t(gridDim.x) * blockDim.x) {
         const int64_t r = idx / out_len;

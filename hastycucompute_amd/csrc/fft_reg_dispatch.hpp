@@ -35,7 +35,7 @@ void launch_one(const PassArgs<V>& a, const char* tag, hipStream_t s) {
         const int64_t n = a.ni * int64_t(a.out_len);
         hipLaunchKernelGGL(sum_partials_kernel<V>, dim3(unsigned(std::min<int64_t>(ceil_div(n, 256), 4096))),
                            dim3(256), 0, s, static_cast<const V*>(a.partial), a.out, int64_t(a.ni), a.out_len,
-                           a.out_i, a.groups, a.accumulate);
+                           a.out_i, a.groups, a.accumulate, int64_t(0));
         HASTY_CHECK_HIP(hipGetLastError());
     }
 }
@@ -181,6 +181,12 @@ bool launch_shape(PassArgs<V> a, int sign, int mode, const char* tag, hipStream_
         const int64_t target = env_blocks > 0 ? env_blocks : 4096;
         const int64_t nchunks = std::min<int64_t>(a.nbatch, std::max<int64_t>(1, ceil_div(target, line_blocks)));
         a.bchunk = int(std::max<int64_t>(1, ceil_div(a.nbatch, nchunks)));
+        // frames: a chunk must not straddle two frames' kernels
+        if (mode == PM_COL_PSF && a.psf_div > 0) {
+            int c = std::min(a.bchunk, a.psf_div);
+            while (a.psf_div % c != 0) --c;
+            a.bchunk = c;
+        }
     }
     if (sign < 0) launch_mode<V, L1, L2, -1>(a, mode, spec, tag, s);
     else launch_mode<V, L1, L2, 1>(a, mode, spec, tag, s);

@@ -64,11 +64,30 @@ static int toep_fused_groups(int64_t nz, int64_t B) {
     return int(std::max<int64_t>(1, std::min<int64_t>(g, B)));
 }
 
-// Core Toeplitz apply on one batch (see file comment).
+// Frames of a batched apply: frame f (f < n) reads input + f*input_f and
+// kernel + f*kernel_f, writes output + f*output_f, and takes its diagonals at
+// ptr + f*sf (DiagOp::sf); each frame has its own B batch elements in the
+// buffers (bufA, bufB hold n*B elements).
+struct ToepFrames {
+    int n = 1;
+    int64_t input_f = 0, kernel_f = 0, output_f = 0;
+};
+
+static DiagOp<float2> diag_at_frame(DiagOp<float2> d, int f) {
+    if (d.ptr) d.ptr += int64_t(f) * d.sf;
+    return d;
+}
+
+// Core Toeplitz apply on one batch (see file comment).  With frames.n > 1
+// and the fused passes, each pass is one launch over all frames (the frame
+// is the second grid axis of the fused passes; the z pass runs the frames'
+// n*B lines with each frame's kernel); otherwise the frames run one after
+// another.
 static void toeplitz_core(const ToeplitzDims& d, int64_t B, const float2* input, float2* output,
                           const float2* kernel, float2* bufA, float2* bufB, DiagOp<float2> in1,
                           DiagOp<float2> in2, DiagOp<float2> out1, DiagOp<float2> out2,
-                          DiagOp<float2> io, int accumulate, int sum_batch, hipStream_t s) {
+                          DiagOp<float2> io, int accumulate, int sum_batch, hipStream_t s,
+                          const ToepFrames& fr = ToepFrames{}) {
     const int64_t nz = d.n[0], ny = d.n[1], nx = d.n[2];
     const int64_t Lz = d.dim == 3 ? 2 * nz : 1, Ly = 2 * ny, Lx = 2 * nx;
     // (one coil per launch: a plane per block leaves half the CUs idle at
@@ -76,12 +95,21 @@ static void toeplitz_core(const ToeplitzDims& d, int64_t B, const float2* input,
     const bool fused = sum_batch && d.dim == 3 && B >= 2 && toep_crop_sum_supported(ny, nx) &&
                        toep_fused_mode() != 0 && !fft_force_stockham();
     const bool fused_first = fused && toep_fused_mode() == 1;
+    HASTY_REQUIRE(fr.n >= 1, "frame count must be >= 1");
+    if (fr.n > 1 && !fused_first) {
+        for (int f = 0; f < fr.n; ++f)
+            toeplitz_core(d, B, input + f * fr.input_f, output + f * fr.output_f, kernel + f * fr.kernel_f, bufA,
+                          bufB, diag_at_frame(in1, f), diag_at_frame(in2, f), diag_at_frame(out1, f),
+                          diag_at_frame(out2, f), diag_at_frame(io, f), accumulate, sum_batch, s);
+        return;
+    }
     if (fused_first) {
         ToepPadFftArgs f{};
         f.in = input; f.in_b = d.nimg();
         f.m1 = in1; f.m2 = in2;
         f.grid = bufB; f.grid_b = d.nB();
         f.nbatch = int(B); f.nz = int(nz); f.n = int(nx);
+        f.nframes = fr.n; f.in_f = fr.input_f; f.grid_f = B * d.nB();
         launch_toep_pad_fft(f, s);
     }
     // P1
@@ -110,7 +138,8 @@ static void toeplitz_core(const ToeplitzDims& d, int64_t B, const float2* input,
         if (!fused_first) launch_col_pass<float2>(c, s);
         // P3 (in place on the first nz planes of bufB)
         ColArgs<float2> z = make_col_args<float2>(int(Lz));
-        z.in = bufB; z.ni = int(Lx); z.no = int(Ly); z.nbatch = int(B);
+        z.in = bufB; z.ni = int(Lx); z.no = int(Ly); z.nbatch = int(B * fr.n);
+        if (fr.n > 1) { z.psf_f = fr.kernel_f; z.psf_div = int(B); }
         z.in_s = Ly * Lx; z.in_o = Lx; z.in_b = nz * Ly * Lx; z.in_len = int(nz); z.in_shift = 0;
         z.out = bufB; z.out_s = Ly * Lx; z.out_o = Lx; z.out_b = nz * Ly * Lx;
         z.out_len = int(nz); z.out_shift = int(nz - 1); z.sign = -1;
@@ -130,6 +159,9 @@ static void toeplitz_core(const ToeplitzDims& d, int64_t B, const float2* input,
             f.groups = toep_fused_groups(nz, B);
             f.accumulate = accumulate;
             f.n = int(nx);
+            f.nframes = fr.n; f.grid_f = B * d.nB(); f.out_f = fr.output_f;
+            f.partial_elems = fr.n * B * d.nA();
+            f.partial_f = int64_t(f.groups) * d.nimg();
             launch_toep_crop_sum(f, s);
             return;
         }
@@ -303,9 +335,14 @@ static void stack_mix(const float2* W, const float2* in, float2* out, int C, int
 //     (the map Psi_j -> stack_j is the same linear map for every j, so the
 //     reference's mix of the outputs equals a mix of the inputs, done first).
 // R_l (kerneldiags) may be absent (all ones).
+// kind 3 (frames): F frames of NORMAL_T_T1_OP sharing the coil maps, frame
+//   f with its own kernel: out[f] = sum_c conj(S_c) (.) T_f(S_c (.) x[f]),
+//   x and out [F, n], kernels [F, 2n...]; every pass is one launch over the
+//   frames and coils (toeplitz_core with frames).
 struct NormalOp {
     ToeplitzDims d;
     int kind = 1;
+    int nframes = 1;
     int nstack = 0, chunk = 0;          // coils / stacked images, and how many per launch
     int nkernels = 1;
     const float2* stack = nullptr;      // kind 1: coil maps S [C, n]
@@ -334,6 +371,24 @@ static void normal_apply(NormalOp* op, const float2* in, float2* out) {
         return o;
     };
     const DiagOp<float2> none{};
+    if (op->kind == 3) {
+        ToepFrames fr;
+        fr.n = op->nframes;
+        fr.input_f = 0;            // the coil maps are shared
+        fr.kernel_f = nK;
+        fr.output_f = nimg;
+        DiagOp<float2> xin = bcast(in, HASTY_MULT);
+        xin.sf = nimg;             // frame f's image
+        for (int c0 = 0; c0 < op->nstack; c0 += op->chunk) {
+            const int cb = std::min(op->chunk, op->nstack - c0);
+            DiagOp<float2> io{};   // (.) conj(S_c), summed over the frame's coils
+            io.ptr = op->out_maps + int64_t(c0) * nimg; io.sb = nimg; io.sr = d.n[2];
+            io.type = HASTY_MULT_CONJ;
+            toeplitz_core(d, cb, op->stack + int64_t(c0) * nimg, out, op->kernels, op->bufA, op->bufB, xin,
+                          none, none, none, io, c0 > 0 ? 1 : 0, 1, op->stream, fr);
+        }
+        return;
+    }
     if (op->kind == 1) {
         for (int l = 0; l < op->nkernels; ++l) {
             const float2* K = op->kernels + int64_t(l) * nK;
@@ -380,15 +435,17 @@ static void* normal_alloc(size_t bytes) {
 static NormalOp* normal_make(int kind, int dim, const int64_t* n, int nstack, const void* stack,
                              const void* x, int nkernels, const void* kernels,
                              const void* kerneldiags, const void* weights, int chunk,
-                             hipStream_t stream) {
+                             hipStream_t stream, int nframes = 1) {
+    HASTY_REQUIRE(nframes >= 1 && nframes <= 65535, "the number of frames must be in [1, 65535]");
     HASTY_REQUIRE(nstack >= 1, "the number of coils / stacked images must be >= 1");
     HASTY_REQUIRE(nkernels >= 1, "the number of kernels must be >= 1");
     HASTY_REQUIRE(kernels != nullptr, "kernels must be non-NULL");
     HASTY_REQUIRE(kind == 2 || stack != nullptr, "smaps must be non-NULL");
-    HASTY_REQUIRE(kind == 1 || x != nullptr, "x must be non-NULL");
+    HASTY_REQUIRE(kind != 2 || x != nullptr, "x must be non-NULL");
     auto op = std::make_unique<NormalOp>();
     op->d = toeplitz_dims(dim, n);
     op->kind = kind;
+    op->nframes = nframes;
     op->nstack = nstack;
     op->chunk = chunk > 0 ? std::min(chunk, nstack) : nstack;
     op->nkernels = nkernels;
@@ -411,8 +468,8 @@ static NormalOp* normal_make(int kind, int dim, const int64_t* n, int nstack, co
             op->mixbuf = static_cast<float2*>(normal_alloc(bytes));
         }
     }
-    op->bufA = static_cast<float2*>(normal_alloc(size_t(op->chunk) * op->d.nA() * 8));
-    op->bufB = static_cast<float2*>(normal_alloc(size_t(op->chunk) * op->d.nB() * 8));
+    op->bufA = static_cast<float2*>(normal_alloc(size_t(nframes) * op->chunk * op->d.nA() * 8));
+    op->bufB = static_cast<float2*>(normal_alloc(size_t(nframes) * op->chunk * op->d.nB() * 8));
     return op.release();
 }
 
@@ -506,6 +563,17 @@ int hasty_normal_create(int dim, const int64_t* n, int ncoils, const void* smaps
         *out = reinterpret_cast<hasty_normal_op>(normal_make(1, dim, n, ncoils, smaps, nullptr, 1, kernel,
                                                              nullptr, nullptr, coil_batch,
                                                              static_cast<hipStream_t>(stream)));
+    });
+}
+
+int hasty_normal_frames_create(int dim, const int64_t* n, int nframes, int ncoils, const void* smaps,
+                               const void* kernels, int coil_batch, void* stream, hasty_normal_op* out) {
+    return guarded([&] {
+        HASTY_REQUIRE(out != nullptr, "output handle pointer is NULL");
+        HASTY_REQUIRE(smaps && kernels, "smaps and kernels must be non-NULL");
+        *out = reinterpret_cast<hasty_normal_op>(normal_make(3, dim, n, ncoils, smaps, nullptr, nframes, kernels,
+                                                             nullptr, nullptr, coil_batch,
+                                                             static_cast<hipStream_t>(stream), nframes));
     });
 }
 

@@ -113,6 +113,15 @@ toep_crop_sum_kernel(ToepCropSumArgs a, const float2* tw) {
     V* R2 = lds + S::R;      // phase B step-1 results
     V* twl = lds + (S::LDS - L);
     const int tid = threadIdx.x;
+    {   // this block's frame
+        const int64_t fr = blockIdx.y;
+        a.grid += fr * a.grid_f;
+        a.out += fr * a.out_f;
+        a.partial += fr * a.partial_f;
+        if (a.m1.ptr) a.m1.ptr += fr * a.m1.sf;
+        if (a.m2.ptr) a.m2.ptr += fr * a.m2.sf;
+        if (a.io.ptr) a.io.ptr += fr * a.io.sf;
+    }
     for (int k = tid; k < L; k += T) twl[k] = tw[k];
 
     const int z = int(blockIdx.x % unsigned(a.nz));
@@ -321,6 +330,13 @@ toep_pad_fft_kernel(ToepPadFftArgs a, const float2* tw, int groups) {
     V* R2 = lds + S::R;      // row exchange
     V* twl = lds + (S::LDS - L);
     const int tid = threadIdx.x;
+    {   // this block's frame
+        const int64_t fr = blockIdx.y;
+        a.in += fr * a.in_f;
+        a.grid += fr * a.grid_f;
+        if (a.m1.ptr) a.m1.ptr += fr * a.m1.sf;
+        if (a.m2.ptr) a.m2.ptr += fr * a.m2.sf;
+    }
     for (int k = tid; k < L; k += T) twl[k] = tw[k];
 
     const int z = int(blockIdx.x % unsigned(a.nz));
@@ -472,7 +488,7 @@ template <int N> void pad_fft_launch(const ToepPadFftArgs& a, hipStream_t s) {
     const int groups = int(std::max<int64_t>(1, std::min<int64_t>((256 + a.nz - 1) / a.nz, a.nbatch)));
     const float2* tw = reinterpret_cast<const float2*>(twiddles(S::L, false));
     HASTY_PROF("toep_F12_xy_pad_fft", s);
-    hipLaunchKernelGGL(kern, dim3(unsigned(int64_t(a.nz) * groups)), dim3(8 * N / kH12),
+    hipLaunchKernelGGL(kern, dim3(unsigned(int64_t(a.nz) * groups), unsigned(a.nframes)), dim3(8 * N / kH12),
                        size_t(S::LDS) * sizeof(float2), s, a, tw, groups);
     HASTY_CHECK_HIP(hipGetLastError());
 }
@@ -481,23 +497,30 @@ template <int N> void crop_sum_launch(const ToepCropSumArgs& a, hipStream_t s) {
     using S = FShape<N>;
     HASTY_REQUIRE(a.grid_b >= int64_t(a.nz) * S::L * S::L, "fused crop pass: grid stride too small");
     const int64_t nimg = int64_t(a.nz) * N * N;
-    if (a.groups > 1)
-        HASTY_REQUIRE(a.partial && a.partial_elems >= int64_t(a.groups) * nimg,
+    if (a.groups > 1) {
+        HASTY_REQUIRE(a.partial_f >= int64_t(a.groups) * nimg || a.nframes == 1,
+                      "fused crop pass: partial-sum frame stride too small");
+        HASTY_REQUIRE(a.partial && a.partial_elems >= int64_t(a.nframes - 1) * a.partial_f + int64_t(a.groups) * nimg,
                       "fused crop pass: partial-sum scratch too small");
+    }
     auto kern = toep_crop_sum_kernel<N, kH45<N>>;
     allow_fused_lds<N>(kern);
     const float2* tw = reinterpret_cast<const float2*>(twiddles(S::L, false));
     {
         HASTY_PROF("toep_F45_yx_ifft_crop_sum", s);
-        hipLaunchKernelGGL(kern, dim3(unsigned(int64_t(a.nz) * a.groups)), dim3(8 * N / kH45<N>),
+        hipLaunchKernelGGL(kern, dim3(unsigned(int64_t(a.nz) * a.groups), unsigned(a.nframes)), dim3(8 * N / kH45<N>),
                            size_t(S::LDS) * sizeof(float2), s, a, tw);
         HASTY_CHECK_HIP(hipGetLastError());
     }
     if (a.groups > 1) {
         HASTY_PROF("toep_F45 sum_partials", s);
-        hipLaunchKernelGGL(sum_partials_kernel<float2>, dim3(unsigned(std::min<int64_t>(ceil_div(nimg, 256), 4096))),
+        // (frame f's partials sit at partial + f * groups * nimg)
+        HASTY_REQUIRE(a.nframes == 1 || a.partial_f == int64_t(a.groups) * nimg,
+                      "fused crop pass: frames' partial sums must be packed");
+        hipLaunchKernelGGL(sum_partials_kernel<float2>,
+                           dim3(unsigned(std::min<int64_t>(ceil_div(nimg, 256), 4096)), unsigned(a.nframes)),
                            dim3(256), 0, s, static_cast<const float2*>(a.partial), a.out, int64_t(a.nz) * N, N,
-                           int64_t(N), a.groups, a.accumulate);
+                           int64_t(N), a.groups, a.accumulate, a.out_f);
         HASTY_CHECK_HIP(hipGetLastError());
     }
 }
@@ -507,6 +530,7 @@ bool toep_crop_sum_supported(int64_t ny, int64_t nx) { return ny == nx && (nx ==
 
 void launch_toep_pad_fft(const ToepPadFftArgs& a, hipStream_t s) {
     HASTY_REQUIRE(a.nz >= 1 && a.nbatch >= 1, "fused pad pass: empty batch or plane count");
+    HASTY_REQUIRE(a.nframes >= 1 && a.nframes <= 65535, "fused pad pass: invalid frame count");
     switch (a.n) {
         case 64: return pad_fft_launch<64>(a, s);
         case 96: return pad_fft_launch<96>(a, s);
@@ -517,6 +541,7 @@ void launch_toep_pad_fft(const ToepPadFftArgs& a, hipStream_t s) {
 
 void launch_toep_crop_sum(const ToepCropSumArgs& a, hipStream_t s) {
     HASTY_REQUIRE(a.nz >= 1 && a.nbatch >= 1, "fused crop pass: empty batch or plane count");
+    HASTY_REQUIRE(a.nframes >= 1 && a.nframes <= 65535, "fused crop pass: invalid frame count");
     HASTY_REQUIRE(a.groups >= 1 && a.groups <= a.nbatch, "fused crop pass: invalid coil group count");
     switch (a.n) {
         case 64: return crop_sum_launch<64>(a, s);

@@ -18,6 +18,10 @@ struct ToepCropSumArgs {
     float scale;
     int nbatch, nz, groups, accumulate;
     int n;                   // plane size ny == nx
+    // frames (blockIdx.y): frame f reads grid + f*grid_f, writes out + f*out_f
+    // and partial + f*partial_f, and its diagonals at ptr + f*sf
+    int nframes = 1;
+    int64_t grid_f = 0, out_f = 0, partial_f = 0;
 };
 
 // First two passes fused: x and y zero-padded forward FFTs of
@@ -30,6 +34,10 @@ struct ToepPadFftArgs {
     int64_t grid_b;
     int nbatch, nz;
     int n;                   // plane size ny == nx
+    // frames (blockIdx.y): frame f reads in + f*in_f and its diagonals at
+    // ptr + f*sf, and writes grid + f*grid_f
+    int nframes = 1;
+    int64_t in_f = 0, grid_f = 0;
 };
 void launch_toep_pad_fft(const ToepPadFftArgs& a, hipStream_t s);
 

@@ -190,6 +190,18 @@ int hasty_normal_idt_create(int dim, const int64_t* n, int ncoils, const void* s
 int hasty_normal_t2_create(int dim, const int64_t* n, int nstack, const void* x, int nkernels,
                            const void* kernels, const void* kerneldiags, const void* weights,
                            int stack_batch, void* stream, hasty_normal_op* op);
+/* ---- batched dynamic frames (no counterpart in the reference, which builds
+ *      one NORMAL_T_T1_OP per frame, normal.cppm:43-93): F frames that share
+ *      the coil maps, frame f with its own kernel,
+ *        out[f] = sum_c conj(S_c) . T_f(S_c . x[f]),
+ *      kernels [F, 2n...]; apply with hasty_normal_apply(op, x [F, n...],
+ *      out [F, n...]).  On the fused 3-D passes (n x n planes, n = 64, 96,
+ *      128, >= 2 coils per launch) every pass is one launch over all frames
+ *      and coils; other shapes run the frames one after another.  Frame f's
+ *      result is bitwise that of hasty_normal_create on kernel f with the
+ *      same coil_batch. */
+int hasty_normal_frames_create(int dim, const int64_t* n, int nframes, int ncoils, const void* smaps,
+                               const void* kernels, int coil_batch, void* stream, hasty_normal_op* op);
 /* x and out must be distinct buffers. */
 int hasty_normal_apply(hasty_normal_op op, const void* x, void* out);
 int hasty_normal_set_stream(hasty_normal_op op, void* stream);
@@ -234,6 +246,15 @@ int hasty_cg_init(void* r, void* p, int64_t n, double tol, double* state, void* 
  * No-op once state.done is set.  */
 int hasty_cg_step(void* x, void* r, void* p, const void* Ap, int64_t n, double* state,
                   void* stream);
+/* hasty_cg_init / hasty_cg_step on nsys independent systems in one launch
+ * per kernel (the batched frames; the reference runs cg.cppm:122-184 once
+ * per system): vectors [nsys, n] (system s at offset s*n), states
+ * [nsys, HASTY_CG_STATE_DOUBLES].  System s gets, bit for bit, what the
+ * single-system call gives on its own vectors; each system stops on its own
+ * done flag. */
+int hasty_cg_init_batched(void* r, void* p, int64_t n, int nsys, double tol, double* state, void* stream);
+int hasty_cg_step_batched(void* x, void* r, void* p, const void* Ap, int64_t n, int nsys, double* state,
+                          void* stream);
 /* Preconditioned CG with restarts (cg.cppm:16-119), as three calls around the
  * caller's A and P applies: z = P(r) comes from the caller between calls.
  *   pcg_init:      rz = Re<r, z>, p <- z, done = sqrt(rz) < tol

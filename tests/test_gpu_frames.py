@@ -2,10 +2,13 @@
 autotuner (hastycucompute_amd.autotune).
 
 * The hipGraph replay of the whole solve (right-hand sides and CG of every
-  frame, as concurrent branches) gives the same result as the eager launches
-  (bitwise: every kernel is deterministic), on three replays with eager work
-  between them; at full size (20 frames of 96^3 x 16 coils) too, with each
-  frame's normal operator Hermitian and positive semi-definite.
+  frame, as concurrent branches or as one batched system of frames) gives
+  the same result as the eager per-frame launches (bitwise: every kernel is
+  deterministic), on three replays with eager work between them; at full
+  size (20 frames of 96^3 x 16 coils) too, with each frame's normal operator
+  Hermitian and positive semi-definite.
+* The batched frames operator (one launch per Toeplitz pass over frames x
+  coils) equals each frame's own normal operator bitwise, and the oracle.
 * Each frame's CG iterate matches the oracle's FP64 CG on the oracle's
   Toeplitz normal operator (same iteration count) to 1e-4.
 * A tuned plan computes the same transform as the default plan.
@@ -48,16 +51,19 @@ def test_frames_graph_equals_eager_and_oracle_cg():
     fb = FrameBatch(cuda(S), [[cuda(c) for c in k] for k in coords], [cuda(y) for y in kdata], tol=1e-6,
                     streams=2)
     iters = 8
-    eager = fb.solve(iters, graph=False, branches=False).clone()
-    eager_br = fb.solve(iters, graph=False, branches=True).clone()
+    eager = fb.solve(iters, graph=False, branches=False, batched=False).clone()
+    eager_br = fb.solve(iters, graph=False, branches=True, batched=False).clone()
     assert torch.equal(eager_br, eager), "eager branches differ from one stream"
+    eager_b = fb.solve(iters, graph=False, batched=True).clone()
+    assert torch.equal(eager_b, eager), "batched frames differ from the per-frame launches"
     for rep in range(3):
-        g = fb.solve(iters, graph=True).clone()
-        torch.cuda.synchronize()
-        _ = (eager * 2).sum().item()            # eager work between the replays
-        assert torch.equal(g, eager), f"graph replay {rep} differs from the eager launches"
-    print(f"[parity] frames: graph == eager bitwise over {iters} iterations x {len(coords)} frames, 3 replays",
-          flush=True)
+        for batched in (False, True):
+            g = fb.solve(iters, graph=True, batched=batched).clone()
+            torch.cuda.synchronize()
+            _ = (eager * 2).sum().item()            # eager work between the replays
+            assert torch.equal(g, eager), f"graph replay {rep} (batched={batched}) differs from the eager launches"
+    print(f"[parity] frames: graph (per-frame and batched) == eager bitwise over {iters} iterations x "
+          f"{len(coords)} frames, 3 replays", flush=True)
     V = float(np.prod(n))
     for f in range(len(coords)):
         k64 = [c.astype(np.float64) for c in coords[f]]
@@ -66,6 +72,41 @@ def test_frames_graph_equals_eager_and_oracle_cg():
                 for c in range(S.shape[0])) / V
         xo = conjugate_gradient(lambda v: sense_normal(v, S, K), b, np.zeros(n, np.complex128), iters)
         report(f"frame {f} ({len(coords[f][0])} pts) CG vs oracle FP64 CG", rel_l2(host(eager[f]), xo), 1e-4)
+
+
+@pytest.mark.parametrize("N,F,C,coil_batch", [(64, 3, 3, 0), (64, 3, 5, 2), (96, 2, 2, 0), (128, 2, 2, 0),
+                                              (20, 3, 3, 0)])
+def test_frames_normal_op_batched_passes(N, F, C, coil_batch):
+    """FramesNormalOp (every Toeplitz pass one launch over frames x coils on
+    the fused 64 / 96 / 128 planes; a 20^3 grid runs the frames one after
+    another) equals the frames' own NORMAL_T_T1_OP applies bitwise -- with
+    coil chunks too, whose last one-coil chunk is not fused -- and frame 0
+    matches the oracle's FP64 normal operator on the same kernel."""
+    from hastycucompute_amd import fft as hf, synth
+    from hastycucompute_amd.frames import FramesNormalOp
+    from hastycucompute_amd.mri import NORMAL_T_T1_OP
+    n = (N, N, N)
+    print(f"[case] frames normal op N={N} F={F} C={C} coil_batch={coil_batch}", flush=True)
+    S = cuda(synth.coil_maps(n, C, seed=4))
+    kernels = torch.empty((F,) + tuple(2 * m for m in n), dtype=torch.complex64, device=S.device)
+    coords = []
+    for f in range(F):
+        k = [cuda(c) for c in synth.rotated_kooshball(60 + 40 * f, 2 * N, f)]
+        hf.toeplitz_kernel(k, kernels[f])
+        coords.append(k)
+    rng = np.random.default_rng(7)
+    x = cuda(crandn(rng, (F,) + n))
+    op = FramesNormalOp(kernels, S, coil_batch=coil_batch)
+    out = op(x)
+    for f in range(F):
+        ref = NORMAL_T_T1_OP(kernels[f], S, coil_batch=coil_batch)(x[f].contiguous())
+        assert torch.equal(out[f], ref), f"frame {f} differs from its own NORMAL_T_T1_OP"
+    if N <= 64:
+        # the oracle's FP64 normal operator on frame 0's kernel (the kernel
+        # itself is pinned against the NDFT PSF in test_gpu_toeplitz.py)
+        K = host(kernels[0]).astype(np.complex128)
+        ref = sense_normal(host(x[0]).astype(np.complex128), host(S).astype(np.complex128), K)
+        report(f"frames op N={N} frame 0 vs oracle FP64 normal operator", rel_l2(host(out[0]), ref), 2e-5)
 
 
 def test_frames_full_size_96c16x20():
@@ -89,9 +130,11 @@ def test_frames_full_size_96c16x20():
         coords.append([torch.from_numpy(c).to(dev) for c in synth.rotated_kooshball(int(ns), samples, f)])
         kdata.append(torch.randn((C, int(ns) * samples), dtype=torch.complex64, device=dev, generator=g))
     fb = FrameBatch(S, coords, kdata, tol=1e-5)
-    eager = fb.solve(iters, graph=False, branches=False).clone()
+    eager = fb.solve(iters, graph=False, branches=False, batched=False).clone()
     times = {}
-    for name, kw in (("eager", dict(graph=False, branches=False)), ("graph", dict(graph=True))):
+    for name, kw in (("eager", dict(graph=False, branches=False, batched=False)),
+                     ("graph", dict(graph=True, batched=False)),
+                     ("batched graph", dict(graph=True, batched=True))):
         fb.solve(iters, **kw)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -101,7 +144,8 @@ def test_frames_full_size_96c16x20():
         assert torch.equal(x, eager), f"{name} solve differs from the eager launches"
     print(f"[workload] frames 20 x 96^3 x 16, {iters} CG iterations + right-hand sides: eager "
           f"{times['eager'] * 1e3:.1f} ms, graph of {fb.nstreams} branches {times['graph'] * 1e3:.1f} ms "
-          f"({times['eager'] / times['graph']:.2f}x), bitwise equal", flush=True)
+          f"({times['eager'] / times['graph']:.2f}x), batched graph {times['batched graph'] * 1e3:.1f} ms "
+          f"({times['eager'] / times['batched graph']:.2f}x), bitwise equal", flush=True)
     for f in (0, 9, 19):
         u = torch.randn(n, dtype=torch.complex64, device=dev, generator=g)
         v = torch.randn(n, dtype=torch.complex64, device=dev, generator=g)
