@@ -499,9 +499,14 @@ __global__ void __launch_bounds__(kLaneSpreadThreads)
     static_assert(W + 6 <= kLaneSpreadPad, "padded weight row too short");
     static_assert(kLaneSpreadChunk % 64 == 0 && kLaneSpreadThreads == 512, "staging: 8 threads per record");
     constexpr int K = kLaneSpreadChunk, PW = kLaneSpreadPad;
+    constexpr int ZROW = K * 3 * PW;   // 16 zeros after the weight rows (padding hits)
     __shared__ int4 lrec[K];
     __shared__ float lval[K * 64];
-    __shared__ float lw[K * 3 * PW];
+    __shared__ float lw[K * 3 * PW + 16];
+    // per wave: the hits of the current 64 records, compacted in visit order
+    // into three runs (both N-tiles, N-tile 0 only, N-tile 1 only), each
+    // padded to an even length, as (value row, z / y / x weight-row starts)
+    __shared__ int4 lhit[8][67];
     int isub, igroup;
     if (!xcd_block(a.nsub, a.ngroups, a.xchunk, isub, igroup)) return;
     const int4 sub = a.subs[isub];
@@ -524,7 +529,7 @@ __global__ void __launch_bounds__(kLaneSpreadThreads)
     // padded weight rows: entries q + 3 for taps q < W are written per chunk,
     // the rest stay zero (the barrier orders these zeros before the first
     // chunk's weights, which land on some of the same entries)
-    for (int e = int(threadIdx.x); e < K * 3 * PW; e += kLaneSpreadThreads) lw[e] = 0.f;
+    for (int e = int(threadIdx.x); e < K * 3 * PW + 16; e += kLaneSpreadThreads) lw[e] = 0.f;
     __syncthreads();
     for (int pass = 0; pass < npass; ++pass) {
         const int sb = pass * 8 + wave;
@@ -595,35 +600,54 @@ __global__ void __launch_bounds__(kLaneSpreadThreads)
                 }
                 // each lane prepares its own record: the offsets of the
                 // point's first support cell from the sub-block origin, in
-                // [-(W - 1), 3] (periodic), as padded-row starts 3 - o, and
-                // which z planes / y rows of the sub-block the support covers
-                int packed = 0;
-                if (hit) {
-                    int oz = rec.y - G0, oy = rec.z - G1, ox = rec.w - G2;
-                    oz += oz < -(W - 1) ? n0 : 0; oz -= oz > 3 ? n0 : 0;
-                    oy += oy < -(W - 1) ? n1 : 0; oy -= oy > 3 ? n1 : 0;
-                    ox += ox < -(W - 1) ? n2 : 0; ox -= ox > 3 ? n2 : 0;
-                    packed = (3 - oz) | ((3 - oy) << 4) | ((3 - ox) << 8);
-                }
-                // hits in visit order, two per MFMA step: lanes 0-31 carry hit
-                // k = 0 and lanes 32-63 hit k = 1 of the pair (an odd last hit
-                // pairs with zero weights)
+                // [-(W - 1), 3] (periodic), as padded-row starts 3 - o: the
+                // LDS indices of its value row and of its z, y, x weights for
+                // the sub-block's first cell along each axis
                 const int k = lane >> 5, j = lane & 31;
                 const int cz = j >> 4, cy = (j >> 2) & 3, cx = j & 3;   // cell j of N-tile 0 (N-tile 1: z + 2)
-                unsigned long long mask = __ballot(hit);
-                while (mask) {
-                    const int i0 = lane_hit_pop(mask);
-                    const int i1 = mask ? lane_hit_pop(mask) : -1;
-                    const int ih = (k == 0 || i1 < 0) ? i0 : i1;
-                    const int pk = __builtin_amdgcn_readlane(packed, i0) * (1 - k) +
-                                   (i1 >= 0 ? __builtin_amdgcn_readlane(packed, i1) : 0) * k;
-                    const float* vr = lval + (rb + ih) * 64;
-                    const float a0 = vr[j], a1 = MT == 2 ? vr[32 + j] : 0.f;
-                    const float* wr = lw + (rb + ih) * 3 * PW;
-                    const float wzr0 = wr[(pk & 15) + cz], wzr1 = wr[(pk & 15) + cz + 2];
-                    const float wyx = wr[PW + ((pk >> 4) & 15) + cy] * wr[2 * PW + ((pk >> 8) & 15) + cx];
-                    const bool live = k == 0 || i1 >= 0;
-                    const float b0 = live ? wzr0 * wyx : 0.f, b1 = live ? wzr1 * wyx : 0.f;
+                // The support meets N-tile 0 (z = 0, 1 of the sub-block) when
+                // oz <= 1 and N-tile 1 (z = 2, 3) when oz >= 3 - W; a hit that
+                // misses one of them skips its two MFMAs.
+                int oz = rec.y - G0, oy = rec.z - G1, ox = rec.w - G2;
+                oz += oz < -(W - 1) ? n0 : 0; oz -= oz > 3 ? n0 : 0;
+                oy += oy < -(W - 1) ? n1 : 0; oy -= oy > 3 ? n1 : 0;
+                ox += ox < -(W - 1) ? n2 : 0; ox -= ox > 3 ? n2 : 0;
+                const bool t0 = hit && oz <= 1, t1 = hit && oz >= 3 - W;
+                const unsigned long long mb = __ballot(t0 && t1), m0 = __ballot(t0 && !t1),
+                                         m1 = __ballot(t1 && !t0);
+                const int nb = __popcll(mb), nt0 = __popcll(m0), nt1 = __popcll(m1);
+                const int base0 = (nb + 1) & ~1, base1 = base0 + ((nt0 + 1) & ~1);
+                int4* hl = lhit[wave];
+                if (hit) {
+                    const unsigned long long mm = t0 && t1 ? mb : (t0 ? m0 : m1);
+                    const int base = t0 && t1 ? 0 : (t0 ? base0 : base1);
+                    const int wrow = (rb + lane) * 3 * PW;
+                    // rank within its run = its run's hits in lower lanes: visit order
+                    const int rank = __builtin_amdgcn_mbcnt_hi(unsigned(mm >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo(unsigned(mm), 0u));
+                    hl[base + rank] = make_int4((rb + lane) * 64, wrow + 3 - oz, wrow + PW + 3 - oy,
+                                                wrow + 2 * PW + 3 - ox);
+                }
+                // an odd run ends with a zero-weight entry (any staged value row)
+                const int4 pad = make_int4(rb * 64, ZROW, ZROW, ZROW);
+                if (lane == 0 && (nb & 1)) hl[nb] = pad;
+                if (lane == 1 && (nt0 & 1)) hl[base0 + nt0] = pad;
+                if (lane == 2 && (nt1 & 1)) hl[base1 + nt1] = pad;
+                __builtin_amdgcn_wave_barrier();
+                // each run's hits in visit order, two per MFMA step: lanes
+                // 0-31 carry hit 2q and lanes 32-63 hit 2q + 1
+                auto load = [&](int e, float& a0, float& a1, float& wz0, float& wz1, float& wyx) {
+                    const int4 h = hl[e + k];
+                    a0 = lval[h.x + j];
+                    a1 = MT == 2 ? lval[h.x + 32 + j] : 0.f;
+                    wz0 = lw[h.y + cz];
+                    wz1 = lw[h.y + cz + 2];
+                    wyx = lw[h.z + cy] * lw[h.w + cx];
+                };
+                for (int q = 0; q < nb; q += 2) {
+                    float a0, a1, wz0, wz1, wyx;
+                    load(q, a0, a1, wz0, wz1, wyx);
+                    const float b0 = wz0 * wyx, b1 = wz1 * wyx;
                     acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, acc[0][0], 0, 0, 0);
                     acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b1, acc[0][1], 0, 0, 0);
                     if constexpr (MT == 2) {
@@ -631,6 +655,24 @@ __global__ void __launch_bounds__(kLaneSpreadThreads)
                         acc[MT - 1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, acc[MT - 1][1], 0, 0, 0);
                     }
                 }
+                for (int q = 0; q < nt0; q += 2) {
+                    float a0, a1, wz0, wz1, wyx;
+                    load(base0 + q, a0, a1, wz0, wz1, wyx);
+                    const float b0 = wz0 * wyx;
+                    acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, acc[0][0], 0, 0, 0);
+                    if constexpr (MT == 2)
+                        acc[MT - 1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b0, acc[MT - 1][0], 0, 0, 0);
+                }
+                for (int q = 0; q < nt1; q += 2) {
+                    float a0, a1, wz0, wz1, wyx;
+                    load(base1 + q, a0, a1, wz0, wz1, wyx);
+                    const float b1 = wz1 * wyx;
+                    acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b1, acc[0][1], 0, 0, 0);
+                    if constexpr (MT == 2)
+                        acc[MT - 1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, acc[MT - 1][1], 0, 0, 0);
+                }
+                // the next group's entries overwrite this list
+                __builtin_amdgcn_wave_barrier();
             }
             __syncthreads();
         }
