@@ -874,6 +874,19 @@ static void* spread_lanes_fn(const NufftPlan* p, int nb) {
     return nufft_spread_lanes_ptr(p->w, p->group);
 }
 
+// Transform pairs per paired-interpolation block: half of the next pair's
+// tile is fetched while the current one is gathered.  Measured at 128^3 x 32
+// coils x 8.4M points: 1 pair 6.78 ms, 2 pairs 6.59, 4 pairs 6.65, 8 pairs
+// 7.22 (fewer, longer blocks balance worse).  HASTY_INTERP_PAIR_RUN
+// overrides (tuning).
+static int interp_pair_run() {
+    static const int env = [] {
+        const char* e = std::getenv("HASTY_INTERP_PAIR_RUN");
+        return e ? std::max(1, std::atoi(e)) : 0;
+    }();
+    return env > 0 ? env : 2;
+}
+
 static int xcd_chunk(bool spread) {
     static const int env = [] {
         const char* e = std::getenv("HASTY_XCD_CHUNK");
@@ -962,7 +975,9 @@ void execute_impl(NufftPlan* p, V* c, V* f, const NufftHooks& hk, int ntransf) {
                 if (p->pair && nb >= 2) {
                     ga.group = 2;
                     ga.ngroups = int(ceil_div(nb, 2));
-                    const int64_t nblocks = xcd_grid(p->nsub_interp, ga.ngroups, ga.xchunk);
+                    ga.pair_run = interp_pair_run();
+                    const int64_t nblocks =
+                        xcd_grid(p->nsub_interp, ceil_div(ga.ngroups, ga.pair_run), ga.xchunk);
                     hipLaunchKernelGGL(reinterpret_cast<GridFn<V>>(nufft_interp_pair_ptr(p->dim, p->w)),
                                        dim3(unsigned(nblocks)), dim3(kInterpPairThreads),
                                        interp_lds_bytes(p, 2), s, ga);

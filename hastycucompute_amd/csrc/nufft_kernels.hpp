@@ -47,6 +47,7 @@ struct GridArgs {
     V* partial;             // [slot][nbatch][tile cells]: partial tiles of split tiles
     int nsub, ngroups;      // subproblems and transform groups (xcd_block)
     int xchunk;             // consecutive subproblems per XCD run (xcd_block)
+    int pair_run;           // paired interpolation: transform pairs per block, run one after another
     const V* pw;            // optional per-point factor [M], caller order (SENSE time segments)
     int accumulate;         // interpolation: add into cdata
     Geo g;
@@ -201,32 +202,41 @@ template <int W, int TZ, int TY, int TX>
 __global__ void __launch_bounds__(kInterpPairThreads) interp_pair_kernel(GridArgs<float2> a) {
     extern __shared__ unsigned char smem_raw[];
     float4* lds = reinterpret_cast<float4*>(smem_raw);
-    int isub, igroup;
-    if (!xcd_block(a.nsub, a.ngroups, a.xchunk, isub, igroup)) return;
+    // a block runs transform pairs [p0, p1) of one subproblem, one after the
+    // other; the XCD mapping numbers its runs like transform groups
+    const int nruns = (a.ngroups + a.pair_run - 1) / a.pair_run;
+    int isub, irun;
+    if (!xcd_block(a.nsub, nruns, a.xchunk, isub, irun)) return;
+    const int p0 = irun * a.pair_run, p1 = min(a.ngroups, p0 + a.pair_run);
     const int4 sub = a.subs[isub];
     int t0[3];
     tile_origin(a.g, sub.x, t0);
     constexpr int wz = TZ == 1 ? 1 : W;
     constexpr int Pz = TZ == 1 ? 1 : TZ + W - 1, Py = TY + W - 1, Px = TX + W - 1;
     constexpr int cells = Pz * Py * Px;
-    const int g0 = 2 * igroup;
-    const bool two = g0 + 1 < a.nbatch;
     const int64_t G = a.g.nf[0] * a.g.nf[1] * a.g.nf[2];
-    const float2* f0 = a.fine + int64_t(g0) * G;
-    const float2* f1 = two ? f0 + G : f0;
-    // stage: every load of the tile in flight at once (one memory latency per
-    // block, which has the CU to itself while it stages).  Thread t stages
-    // padded cells t + 1024 u: its first cell's (pz, py, px) once, then the
-    // compile-time steps of 1024 cells with their carries; the fine grid
-    // wraps with one unsigned min per axis (t0 + p < 2 nf) and the offsets
-    // are 32-bit byte offsets (the plan pairs transforms only when one
-    // transform's fine grid is under 4 GiB)
+    // Staging: thread t stages padded cells t + 1024 u of a pair's tile: its
+    // first cell's (pz, py, px) once, then the compile-time steps of 1024
+    // cells with their carries; the fine grid wraps with one unsigned min per
+    // axis (t0 + p < 2 nf) and the offsets are 32-bit byte offsets (the plan
+    // pairs transforms only when one transform's fine grid is under 4 GiB).
+    // The loads of the first half of pair p + 1's cells are issued right after
+    // the point data of pair p's first round, so they stream in while pair p
+    // is gathered; the second half is loaded when pair p is done, then the
+    // whole tile goes to LDS.
     constexpr int SU = (cells + kInterpPairThreads - 1) / kInterpPairThreads;
-    {
+    // cells held in registers across a gather: the first SP steps (all of
+    // them would spill at the 128-VGPR cap of 1024-thread blocks)
+    constexpr int SP = SU / 2;
+    float2 v0[SU], v1[SU];
+    auto fetch = [&](int pair, int ulo, int uhi) {
+        const int g0 = 2 * pair;
+        const bool two = g0 + 1 < a.nbatch;
+        const float2* f0 = a.fine + int64_t(g0) * G;
+        const float2* f1 = two ? f0 + G : f0;
         const int base = int(threadIdx.x);
         const unsigned nf0 = unsigned(a.g.nf[0]), nf1 = unsigned(a.g.nf[1]), nf2 = unsigned(a.g.nf[2]);
         const int px0 = base % Px, py0 = (base / Px) % Py, pz0 = base / (Px * Py);
-        float2 v0[SU], v1[SU];
 #pragma unroll
         for (int u = 0; u < SU; ++u) {
             constexpr int kStep = kInterpPairThreads;
@@ -234,7 +244,7 @@ __global__ void __launch_bounds__(kInterpPairThreads) interp_pair_kernel(GridArg
             int px = px0 + dx, py = py0 + dy, pz = pz0 + dz;
             if (px >= Px) { px -= Px; ++py; }
             if (py >= Py) { py -= Py; ++pz; }
-            if (base + u * kStep < cells) {
+            if (u >= ulo && u < uhi && base + u * kStep < cells) {
                 unsigned gz = unsigned(t0[0] + pz), gy = unsigned(t0[1] + py), gx = unsigned(t0[2] + px);
                 gz = min(gz, gz - nf0);
                 gy = min(gy, gy - nf1);
@@ -246,57 +256,73 @@ __global__ void __launch_bounds__(kInterpPairThreads) interp_pair_kernel(GridArg
                 if (!two) v1[u] = make_float2(0.f, 0.f);
             }
         }
+    };
+    fetch(p0, 0, SP);
+    for (int pair = p0; pair < p1; ++pair) {
+        const int g0 = 2 * pair;
+        const bool two = g0 + 1 < a.nbatch;
+        fetch(pair, SP, SU);
+        if (pair > p0) __syncthreads();   // the previous pair's gather is done with the tile
+        {
+            const int base = int(threadIdx.x);
 #pragma unroll
-        for (int u = 0; u < SU; ++u)
-            if (base + u * kInterpPairThreads < cells)
-                lds[base + u * kInterpPairThreads] = make_float4(v0[u].x, v0[u].y, v1[u].x, v1[u].y);
-    }
-    __syncthreads();
-    for (int s = sub.y + threadIdx.x; s < sub.y + sub.z; s += blockDim.x) {
-        const int lz = a.sl0[s] - t0[0];
-        const int ly = a.sl0[a.M + s] - t0[1];
-        const int lx = a.sl0[2 * a.M + s] - t0[2];
-        const float* w = a.sw + int64_t(s) * a.ws;
-        float wz_[W], wy[W], wx[W];
-#pragma unroll
-        for (int q = 0; q < W; ++q) {
-            wz_[q] = w[q];
-            wy[q] = w[W + q];
-            wx[q] = w[2 * W + q];
+            for (int u = 0; u < SU; ++u)
+                if (base + u * kInterpPairThreads < cells)
+                    lds[base + u * kInterpPairThreads] = make_float4(v0[u].x, v0[u].y, v1[u].x, v1[u].y);
         }
-        const int orig = a.sidx[s];
-        const float4* tb = lds + (lz * Py + ly) * Px + lx;
-        float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-        // one z plane of the window per iteration: unrolled whole, its 16-B
-        // reads (4 VGPRs each) would be hoisted together and spill
-#pragma unroll 1
-        for (int qz = 0; qz < wz; ++qz) {
-            float4 accy = make_float4(0.f, 0.f, 0.f, 0.f);
+        __syncthreads();
+        bool fetched = pair + 1 >= p1;
+        for (int s = sub.y + threadIdx.x; s < sub.y + sub.z; s += blockDim.x) {
+            const int lz = a.sl0[s] - t0[0];
+            const int ly = a.sl0[a.M + s] - t0[1];
+            const int lx = a.sl0[2 * a.M + s] - t0[2];
+            const float* w = a.sw + int64_t(s) * a.ws;
+            float wz_[W], wy[W], wx[W];
 #pragma unroll
-            for (int qy = 0; qy < W; ++qy) {
-                float4 row[W];
-#pragma unroll
-                for (int qx = 0; qx < W; ++qx) row[qx] = tb[(qz * Py + qy) * Px + qx];
-                float4 accx = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-                for (int qx = 0; qx < W; ++qx) {
-                    accx.x += wx[qx] * row[qx].x;
-                    accx.y += wx[qx] * row[qx].y;
-                    accx.z += wx[qx] * row[qx].z;
-                    accx.w += wx[qx] * row[qx].w;
-                }
-                accy.x += wy[qy] * accx.x;
-                accy.y += wy[qy] * accx.y;
-                accy.z += wy[qy] * accx.z;
-                accy.w += wy[qy] * accx.w;
+            for (int q = 0; q < W; ++q) {
+                wz_[q] = w[q];
+                wy[q] = w[W + q];
+                wx[q] = w[2 * W + q];
             }
-            acc.x += wz_[qz] * accy.x;
-            acc.y += wz_[qz] * accy.y;
-            acc.z += wz_[qz] * accy.z;
-            acc.w += wz_[qz] * accy.w;
+            const int orig = a.sidx[s];
+            if (!fetched) {
+                fetch(pair + 1, 0, SP);
+                fetched = true;
+            }
+            const float4* tb = lds + (lz * Py + ly) * Px + lx;
+            float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+            // one z plane of the window per iteration: unrolled whole, its 16-B
+            // reads (4 VGPRs each) would be hoisted together and spill
+#pragma unroll 1
+            for (int qz = 0; qz < wz; ++qz) {
+                float4 accy = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+                for (int qy = 0; qy < W; ++qy) {
+                    float4 row[W];
+#pragma unroll
+                    for (int qx = 0; qx < W; ++qx) row[qx] = tb[(qz * Py + qy) * Px + qx];
+                    float4 accx = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+                    for (int qx = 0; qx < W; ++qx) {
+                        accx.x += wx[qx] * row[qx].x;
+                        accx.y += wx[qx] * row[qx].y;
+                        accx.z += wx[qx] * row[qx].z;
+                        accx.w += wx[qx] * row[qx].w;
+                    }
+                    accy.x += wy[qy] * accx.x;
+                    accy.y += wy[qy] * accx.y;
+                    accy.z += wy[qy] * accx.z;
+                    accy.w += wy[qy] * accx.w;
+                }
+                acc.x += wz_[qz] * accy.x;
+                acc.y += wz_[qz] * accy.y;
+                acc.z += wz_[qz] * accy.z;
+                acc.w += wz_[qz] * accy.w;
+            }
+            store_point(a, int64_t(g0), orig, make_float2(acc.x, acc.y));
+            if (two) store_point(a, int64_t(g0 + 1), orig, make_float2(acc.z, acc.w));
         }
-        store_point(a, int64_t(g0), orig, make_float2(acc.x, acc.y));
-        if (two) store_point(a, int64_t(g0 + 1), orig, make_float2(acc.z, acc.w));
+        if (!fetched) fetch(pair + 1, 0, SP);   // (no points for this thread)
     }
 }
 
