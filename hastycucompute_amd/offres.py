@@ -99,8 +99,12 @@ def make_offresonance_interpolators(fieldmap: torch.Tensor, times: torch.Tensor,
     keep = h > 0
     zk, w = zk[keep], h[keep].sqrt().to(torch.complex128)
     ch = torch.exp(zk.view(1, -1) * tl.view(-1, 1))                    # [L, K]
-    rhs = w.view(-1, 1) * torch.exp(zk.view(-1, 1) * t.view(1, -1))     # [K, M]
-    B = torch.linalg.pinv(w.view(-1, 1) * ch.T) @ rhs                   # [L, M]
+    # the fit of column j depends on t_j alone, and a trajectory repeats its
+    # readout times (every spoke or interleave has the same ones): fit each
+    # distinct time once and gather
+    tu, inv = torch.unique(t, return_inverse=True)
+    rhs = w.view(-1, 1) * torch.exp(zk.view(-1, 1) * tu.view(1, -1))    # [K, U]
+    B = (torch.linalg.pinv(w.view(-1, 1) * ch.T) @ rhs)[:, inv]        # [L, M]
     z = fieldmap.detach().to(torch.complex128)
     Ct = torch.exp(z.unsqueeze(0) * tl.view((L,) + (1,) * z.dim()))
     return B.to(dtype).contiguous(), Ct.to(dtype).contiguous()

@@ -179,6 +179,28 @@ def test_offres_error_falls_with_segments_and_matches_oracle():
     assert errs[0] > errs[1] > errs[2] and errs[2] < 1e-2, errs
 
 
+def test_offres_repeated_unsorted_times_match_oracle():
+    """Readout times that repeat (every spoke has the same ones) and come
+    unsorted: each column of B is the fit at its own time, as the oracle
+    computes it column by column."""
+    import torch
+    from hastycucompute_amd.offres import make_offresonance_interpolators
+    from oracle.offres import interpolators
+    rng = np.random.default_rng(5)
+    shape = (6, 7, 5)
+    fmap = -15.0 + 1j * 2 * np.pi * 40.0 * rng.uniform(-1, 1, shape)
+    readout = rng.uniform(0.0, 6e-3, 37)
+    t = readout[rng.integers(0, 37, 600)]               # 600 samples, 37 distinct times, unsorted
+    B, Ct = make_offresonance_interpolators(torch.from_numpy(fmap), torch.from_numpy(t), 4, 30,
+                                            dtype=torch.complex128)
+    Bo, Cto = interpolators(fmap, t, 4, 30)
+    assert B.shape == (4, 600)
+    assert np.linalg.norm(B.numpy() - Bo) <= 1e-6 * np.linalg.norm(Bo)
+    # equal times give equal columns
+    j0, j1 = np.nonzero(t == t[0])[0][:2]
+    assert np.array_equal(B.numpy()[:, j0], B.numpy()[:, j1])
+
+
 def test_offres_autocorrelated_histogram():
     import torch
     from hastycucompute_amd.offres import histogram
