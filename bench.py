@@ -321,7 +321,7 @@ _SQ_PASSES = ("SQ_INSTS_VALU SQ_INSTS_SALU SQ_INSTS_SMEM SQ_INSTS_VMEM_RD SQ_WAI
               "SQ_WAVES SQ_INSTS_VALU_FMA_F32 SQ_INSTS_LDS SQ_LDS_BANK_CONFLICT SQ_ACTIVE_INST_LDS GRBM_GUI_ACTIVE",
               "SQ_INSTS_MFMA SQ_VALU_MFMA_BUSY_CYCLES SQ_INSTS_VMEM_WR SQ_WAIT_ANY")
 _SQ_KERNELS = (("spread_lanes_kernel<6, 32>", "spread_lanes_kernel<6, 32>", 10_000_000),
-               ("interp_pair_kernel<6, 16, 16, 16>", "interp_pair_kernel<6, 16, 16, 16>", 40_000_000))
+               ("interp_pair_kernel<6, 16, 16, 16>", "interp_pair_kernel<6, 16, 16, 16>", 20_000_000))
 
 
 # the Toeplitz passes of the A^H A apply (full-size launches: the grid of the
