@@ -705,7 +705,9 @@ __global__ void __launch_bounds__(kLaneSpreadThreads)
         if (!active) continue;
         // ---- write the sub-block's cells of this tile ----
         // 32 x 32 MFMA output: register v of lane l holds row
-        // 8 (v / 4) + 4 (l / 32) + v % 4 and column l % 32 of its tile.
+        // 8 (v / 4) + 4 (l / 32) + v % 4 and column l % 32 of its tile, so
+        // registers v and v + 1 (v even) hold the real and imaginary parts of
+        // one transform at one cell: each lane stores whole complex values.
         // (An opaque copy of the lane index keeps the compiler from hoisting
         // these 64 addresses out of the loops into registers.)
         int lw_ = lane;
@@ -715,19 +717,19 @@ __global__ void __launch_bounds__(kLaneSpreadThreads)
 #pragma unroll
             for (int n = 0; n < 2; ++n)
 #pragma unroll
-                for (int v = 0; v < 16; ++v) {
-                    const int tp = 32 * m + 8 * (v >> 2) + 4 * (lw_ >> 5) + (v & 3);   // transform-part
+                for (int v = 0; v < 16; v += 2) {
+                    const int tr = 16 * m + 4 * (v >> 2) + 2 * (lw_ >> 5) + ((v & 3) >> 1);   // transform
                     const int cell_sb = 32 * n + (lw_ & 31);
                     const int lz = o0 + (cell_sb >> 4), ly = o1 + ((cell_sb >> 2) & 3), lx = o2 + (cell_sb & 3);
-                    if ((tp >> 1) >= ng || lz >= T0 || ly >= T1 || lx >= T2) continue;
+                    if (tr >= ng || lz >= T0 || ly >= T1 || lx >= T2) continue;
                     const int cell = (lz * T1 + ly) * T2 + lx;
-                    float* dst;
+                    float2* dst;
                     if (sub.w != 0)
-                        dst = reinterpret_cast<float*>(a.partial + (int64_t(sub.w - 1) * a.nbatch + g0 + (tp >> 1)) * cells + cell);
+                        dst = a.partial + (int64_t(sub.w - 1) * a.nbatch + g0 + tr) * cells + cell;
                     else
-                        dst = reinterpret_cast<float*>(a.fine + int64_t(g0 + (tp >> 1)) * G +
-                                                       (int64_t(t0[0] + lz) * n1 + (t0[1] + ly)) * n2 + (t0[2] + lx));
-                    dst[tp & 1] = acc[m][n][v];
+                        dst = a.fine + int64_t(g0 + tr) * G +
+                              (int64_t(t0[0] + lz) * n1 + (t0[1] + ly)) * n2 + (t0[2] + lx);
+                    *dst = make_float2(acc[m][n][v], acc[m][n][v + 1]);
                 }
     }
 }
