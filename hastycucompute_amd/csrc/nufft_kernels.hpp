@@ -617,27 +617,28 @@ __global__ void __launch_bounds__(kLaneSpreadThreads)
             }
             const int nrec = min(K, vend - vb);
             for (int rb = 0; active && rb < nrec; rb += 64) {
-                int4 rec = make_int4(0, 0, 0, 0);
-                bool hit = false;
-                if (rb + lane < nrec) {
-                    rec = lrec[rb + lane];
-                    hit = arcs_meet(rec.y, W, G0, 4, n0) && arcs_meet(rec.z, W, G1, 4, n1) &&
-                          arcs_meet(rec.w, W, G2, 4, n2);
-                }
-                // each lane prepares its own record: the offsets of the
-                // point's first support cell from the sub-block origin, in
-                // [-(W - 1), 3] (periodic), as padded-row starts 3 - o: the
-                // LDS indices of its value row and of its z, y, x weights for
-                // the sub-block's first cell along each axis
+                // each lane takes one record: the offsets of the point's first
+                // support cell from the sub-block origin, wrapped once per axis
+                // (each fine-grid axis is >= W + 4); the support meets the
+                // sub-block exactly when all three lie in [-(W - 1), 3].  The
+                // test is branch-free; lanes past the chunk's records (stale
+                // LDS entries, rb + lane < K) are masked out.
+                const int4 rec = lrec[rb + lane];
+                int oz = rec.y - G0, oy = rec.z - G1, ox = rec.w - G2;
+                oz += oz < -(W - 1) ? n0 : 0; oz -= oz > 3 ? n0 : 0;
+                oy += oy < -(W - 1) ? n1 : 0; oy -= oy > 3 ? n1 : 0;
+                ox += ox < -(W - 1) ? n2 : 0; ox -= ox > 3 ? n2 : 0;
+                const bool hit = (rb + lane < nrec) & (unsigned(oz + (W - 1)) <= unsigned(W + 2)) &
+                                 (unsigned(oy + (W - 1)) <= unsigned(W + 2)) &
+                                 (unsigned(ox + (W - 1)) <= unsigned(W + 2));
+                // as padded-row starts 3 - o: the LDS indices of the point's
+                // value row and of its z, y, x weights for the sub-block's
+                // first cell along each axis
                 const int k = lane >> 5, j = lane & 31;
                 const int cz = j >> 4, cy = (j >> 2) & 3, cx = j & 3;   // cell j of N-tile 0 (N-tile 1: z + 2)
                 // The support meets N-tile 0 (z = 0, 1 of the sub-block) when
                 // oz <= 1 and N-tile 1 (z = 2, 3) when oz >= 3 - W; a hit that
                 // misses one of them skips its two MFMAs.
-                int oz = rec.y - G0, oy = rec.z - G1, ox = rec.w - G2;
-                oz += oz < -(W - 1) ? n0 : 0; oz -= oz > 3 ? n0 : 0;
-                oy += oy < -(W - 1) ? n1 : 0; oy -= oy > 3 ? n1 : 0;
-                ox += ox < -(W - 1) ? n2 : 0; ox -= ox > 3 ? n2 : 0;
                 const bool t0 = hit && oz <= 1, t1 = hit && oz >= 3 - W;
                 const unsigned long long mb = __ballot(t0 && t1), m0 = __ballot(t0 && !t1),
                                          m1 = __ballot(t1 && !t0);
