@@ -252,6 +252,47 @@ def test_many_coil_kooshball_vs_kb_oracle(ntransf, kw):
     report(f"kooshball C={ntransf} {kw} type1 vs KB oracle", rel_l2(g1, kb.type1(c)), 2e-6)
 
 
+@pytest.mark.parametrize("ntransf,kw", [(10, {}), (20, {}), (34, {}), (40, {"max_batch": 18}), (17, {})])
+def test_spread_partial_transform_groups(ntransf, kw):
+    """3-D complex64 spreading takes the plan's transform group (the next power
+    of two covering the batch, at most 32) per block, on the MFMA kernel for
+    groups of 16 or 32 and even batches.  Batches that leave a group partly
+    filled (10 of 16, 20 of 32, 34 = 32 + 2, chunks of 18, 18 and 4 of 32)
+    read whole value rows and store only their own transforms: each transform
+    equals, bit for bit, the same transform in a full plan of the same group
+    (16 or 32 transforms, independent of one another), and all match the KB
+    oracle.
+    An odd batch (17) runs the cell-per-lane kernel: oracle only."""
+    from hastycucompute_amd import fft as hf, synth
+    shape = (32, 32, 32)
+    coords = synth.kooshball(256, 64)
+    M = len(coords[0])
+    gco = [cuda(x) for x in coords]
+    rng = np.random.default_rng(ntransf)
+    c = crandn(rng, (ntransf, M))
+    print(f"[case] partial transform groups ntransf={ntransf} {kw}", flush=True)
+
+    def type1(vals, T, **opts):
+        p = hf.nufft_plan.make(hf.nufft_opts(nmodes=shape, ntransf=T, tol=EPS, **opts), hf.nufft_type.TYPE_1)
+        p.setpts(gco)
+        out = torch.empty((T,) + shape, dtype=torch.complex64, device="cuda")
+        p.execute(cuda(vals), out)
+        p.free()
+        return out
+
+    got = type1(c, ntransf, **kw)
+    if ntransf % 2 == 0:
+        G = 16 if ntransf <= 16 else 32
+        for lo in range(0, ntransf, G):
+            hi = min(ntransf, lo + G)
+            full = np.zeros((G, M), np.complex64)
+            full[:hi - lo] = c[lo:hi]
+            ref = type1(full, G)
+            assert torch.equal(got[lo:hi], ref[:hi - lo]), f"transforms {lo}..{hi - 1} differ from the {G}-transform plan"
+    kb = KBPlan(shape, EPS).setpts(coords)
+    report(f"partial groups C={ntransf} {kw} type1 vs KB oracle", rel_l2(host(got), kb.type1(c)), 2e-6)
+
+
 def test_setpts_is_deterministic():
     """The sorted point order and the visit lists come from stable sorts, so
     the same trajectory gives bitwise the same transforms on every setpts."""
