@@ -153,9 +153,10 @@ def cpu_baseline_nufft(N, C, coords_np, smaps_np, x_np, sample_coils):
             "seconds": {"type2": round(t2, 3), "type1": round(t1, 3)}}
 
 
-def cpu_baseline(N, C, smaps_np, x_np, kernel_np, sample_coils):
+def cpu_baseline(N, C, smaps_np, x_np, kernel_np, sample_coils, reps):
     """Time the oracle's CPU restatement of the Toeplitz normal operator on a
-    bounded sample of coils (complex64, scipy.fft on every host core)."""
+    bounded sample (``reps`` applies over ``sample_coils`` coils; complex64,
+    scipy.fft on every host core)."""
     import numpy as np
 
     from oracle.solvers import sense_normal
@@ -163,7 +164,6 @@ def cpu_baseline(N, C, smaps_np, x_np, kernel_np, sample_coils):
     os.environ.setdefault("OMP_NUM_THREADS", str(cores))
     S = smaps_np[:sample_coils]
     sense_normal(x_np, S[:1], kernel_np, dtype=np.complex64, workers=cores)   # warm
-    reps = 2
     t0 = time.perf_counter()
     for _ in range(reps):
         sense_normal(x_np, S, kernel_np, dtype=np.complex64, workers=cores)
@@ -171,7 +171,9 @@ def cpu_baseline(N, C, smaps_np, x_np, kernel_np, sample_coils):
     per_apply = t * C / sample_coils
     return {"value": 1.0 / per_apply, "unit": "applies/s", "cores": cores, "kind": "port",
             "sample": f"oracle Toeplitz normal op (complex64, scipy.fft, {cores} threads) on "
-                      f"{sample_coils} of {C} coils at {N}^3, {reps} reps, extrapolated x{C / sample_coils:g}",
+                      + (f"all {C} coils" if sample_coils == C else f"{sample_coils} of {C} coils")
+                      + f" at {N}^3, {reps} applies"
+                      + ("" if sample_coils == C else f", extrapolated x{C / sample_coils:g}"),
             "seconds_per_apply": per_apply}
 
 
@@ -501,8 +503,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-coils", type=int, default=2)
-    ap.add_argument("--cpu-nufft-coils", type=int, default=4)
+    # CPU baselines: about 10 s of host work each (a full 32-coil apply takes
+    # ~1 s on 16 cores; a type 2 + type 1 of 8 coils ~14 s)
+    ap.add_argument("--cpu-sample-coils", type=int, default=32)
+    ap.add_argument("--cpu-reps", type=int, default=10)
+    ap.add_argument("--cpu-nufft-coils", type=int, default=8)
     ap.add_argument("--phases", default="aha,type2,type1,plain,segments,frames")
     ap.add_argument("--reduce-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="reduce two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of this script "
@@ -785,7 +790,8 @@ def main():
     cpu_nufft = None
     if not args.no_cpu_baseline and "aha" in phases:
         log("[bench] CPU baseline ...")
-        cpu = cpu_baseline(N, C, smaps_np, x_np, kernel.cpu().numpy(), max(1, min(args.cpu_sample_coils, C)))
+        cpu = cpu_baseline(N, C, smaps_np, x_np, kernel.cpu().numpy(), max(1, min(args.cpu_sample_coils, C)),
+                           max(1, args.cpu_reps))
         log(f"[bench] CPU baseline {cpu['value']:.4f} applies/s")
     if not args.no_cpu_baseline and ("type1" in phases or "type2" in phases):
         log("[bench] CPU NUFFT baseline ...")
