@@ -13,8 +13,9 @@
 // j = n' the modes n' - 64 < 0).  Per plane z, coil b and class r:
 //   Y  each column x: fold, 64-point DFT (8 x 8, one LDS exchange) -> rows m;
 //   X  each row m: half-input 256-point DFT along x (16 x 16, one
-//      exchange), times e^{-s pi i kx / 2} for the -64 shift, staged in LDS
-//      and stored as whole 2-KB fine-grid rows ky = 4m + r.
+//      exchange), times e^{-s pi i kx / 2} for the -64 shift; the second
+//      step runs 16 lanes per row (lane k1 holds kx = k1 + 16 k2), so each
+//      store instruction writes 128-B runs of 4 fine-grid rows ky = 4m + r.
 #include <algorithm>
 
 #include "fft_reg.hpp"
@@ -28,9 +29,9 @@ constexpr int kT = 1024;
 constexpr int kR2 = 72 * kN;           // y exchange [s 9 + k1][x]
 constexpr int kRS1 = kN + 1;           // class rows [m][x], odd stride
 constexpr int kR1 = kM * kRS1;
-constexpr int kOS = kL + 1;            // staged output rows [m][kx]
-constexpr int kE = 16 * 16 * kM;       // x exchange [k1][p2][m]
-constexpr int kLdsElems = std::max(kR2 + kR1, std::max(kE, kM * kOS)) + kL;
+constexpr int kES = 17;                // x exchange [p2][m][k1], k1 padded to 17
+constexpr int kE = 16 * kM * kES;
+constexpr int kLdsElems = std::max(kR2 + kR1, kE) + kL;
 }  // namespace
 
 template <int SIGN>
@@ -42,7 +43,6 @@ nufft_fwd_xy_kernel(NufftFwdXyArgs a, const float2* tw) {
     V* R2 = lds;               // y exchange
     V* R1 = lds + kR2;         // class rows
     V* E = lds;                // x exchange (R1 / R2 are read by then)
-    V* O = lds;                // staged output rows (E is read by then)
     V* twl = lds + (kLdsElems - kL);
     const int tid = threadIdx.x;
     for (int k = tid; k < kL; k += kT) twl[k] = tw[k];
@@ -118,7 +118,7 @@ nufft_fwd_xy_kernel(NufftFwdXyArgs a, const float2* tw) {
             RegDFT<8, SIGN, V>::run(u);
 #pragma unroll
             for (int k1 = 1; k1 < 8; ++k1) u[k1] = twiddle_mul<V, SIGN>(u[k1], twl, (4 * s * k1) & (kL - 1));
-            __syncthreads();   // the previous class's staged rows are stored
+            __syncthreads();   // the previous class's exchange reads are done
 #pragma unroll
             for (int k1 = 0; k1 < 8; ++k1) R2[(s * 9 + k1) * kN + x] = u[k1];
             __syncthreads();
@@ -136,26 +136,31 @@ nufft_fwd_xy_kernel(NufftFwdXyArgs a, const float2* tw) {
 #pragma unroll
             for (int k1 = 1; k1 < 16; ++k1) v[k1] = twiddle_mul<V, SIGN>(v[k1], twl, (pa * k1) & (kL - 1));
             __syncthreads();   // R1 is read; E overwrites it
+            // (lanes over rows mm: stride 17 keeps the 8-B writes conflict-free)
 #pragma unroll
-            for (int k1 = 0; k1 < 16; ++k1) E[(k1 * 16 + pa) * kM + mm] = v[k1];
+            for (int k1 = 0; k1 < 16; ++k1) E[(pa * kM + mm) * kES + k1] = v[k1];
             __syncthreads();
-#pragma unroll
-            for (int p2 = 0; p2 < 16; ++p2) v[p2] = E[(pa * 16 + p2) * kM + mm];
-            RegDFT<16, SIGN, V>::run(v);   // v[k2] = kx = pa + 16 k2, before the shift
-            // the -64 shift: output kx times e^{-s pi i kx / 2} = i^{-s (kx mod 4)}, kx mod 4 = pa mod 4
-            const int q = SIGN < 0 ? (pa & 3) : ((4 - (pa & 3)) & 3);
-#pragma unroll
-            for (int k2 = 0; k2 < 16; ++k2) v[k2] = quarter_turn(v[k2], q);
-            __syncthreads();   // E is read; O overwrites it
-#pragma unroll
-            for (int k2 = 0; k2 < 16; ++k2) O[mm * kOS + pa + 16 * k2] = v[k2];
-            __syncthreads();
-            // ---- store: whole fine-grid rows ky = 4m + r ----
             {
-                const int kx = tid & (kL - 1), m4 = tid >> 8;   // 4 rows per round
-#pragma unroll 1
-                for (int m = m4; m < kM; m += kT / kL)
-                    store_at<V, true>(gp, uint32_t(((4 * m + r) * kL + kx) * int(sizeof(V))), O[m * kOS + kx]);
+                // second step: lane k1 of row mr gathers p2 = 0..15
+                int k1 = tid & 15, mr = tid >> 4;
+                asm volatile("" : "+v"(k1), "+v"(mr));
+#pragma unroll
+                for (int p2 = 0; p2 < 16; ++p2) v[p2] = E[(p2 * kM + mr) * kES + k1];
+                RegDFT<16, SIGN, V>::run(v);   // v[k2] = kx = k1 + 16 k2, before the shift
+                // the -64 shift: output kx times e^{-s pi i kx / 2} = i^{-s (kx mod 4)}, kx mod 4 = k1 mod 4
+                // (lane-varying: a multiply by i, then a negation, as selects)
+                const int q = SIGN < 0 ? (k1 & 3) : ((4 - (k1 & 3)) & 3);
+                const uint32_t row = uint32_t(((4 * mr + r) * kL + k1) * int(sizeof(V)));
+#pragma unroll
+                for (int k2 = 0; k2 < 16; ++k2) {
+                    float re = v[k2].x, im = v[k2].y;
+                    const float t = re;
+                    re = (q & 1) ? -im : re;
+                    im = (q & 1) ? t : im;
+                    re = (q & 2) ? -re : re;
+                    im = (q & 2) ? -im : im;
+                    store_at<V, true>(gp, row + uint32_t(16 * k2 * int(sizeof(V))), make_float2(re, im));
+                }
             }
         }
     }
