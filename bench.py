@@ -254,7 +254,7 @@ _TIMED_SEQ = ([("toep_F12_xy_pad_fft", "toep_pad_fft_kernel"),
                ("fwd_z_pad_fft", "<c64, 16, 16, -1, 0, 1>"), ("nufft_interp", "interp_pair_kernel<6,")] +
               [("nufft_sort_values", "sort_values_kernel<c64"), ("nufft_spread", "spread_lanes_kernel<6, 32>"),
                ("nufft_spread_reduce", "reduce_split_tiles"), ("inv_z_fft_crop", "<c64, 16, 16, 1, 0, 4>"),
-               ("inv_y_fft_crop", "<c64, 16, 16, 1, 0, 4>"), ("inv_x_fft_crop", "<c64, 16, 16, 1, 3, 4>")]) * 3
+               ("inv_xy_fft_crop", "nufft_inv_xy_kernel")]) * 3
 
 
 def _short_kernel(name):

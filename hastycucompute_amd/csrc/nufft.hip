@@ -42,8 +42,9 @@ template <class V> int rows_per_block(int L, int out_len) {
 }
 
 // The x and y passes of 3-D c64 plans with 128 x 128 planes on a 256 x 256
-// fine grid run as one fused kernel (nufft_fused.hip); HASTY_NUFFT_FUSED=0
-// keeps the separate passes.
+// fine grid run as one fused kernel (nufft_fused.hip), forward and inverse
+// (the inverse without a batch sum); HASTY_NUFFT_FUSED=0 keeps the separate
+// passes.
 static bool nufft_fused_enabled() {
     static const bool v = [] {
         const char* e = std::getenv("HASTY_NUFFT_FUSED");
@@ -129,6 +130,26 @@ void pruned_inverse(const PadGeometry& g, int64_t B, const V* grid, V* img, V* b
         z.tag = "inv_z_fft_crop";
         launch_col_pass<V>(z, s);
         ysrc = bufB;
+        // the y and x passes as one fused kernel (nufft_fused.hip) for 3-D
+        // c64 plans with 128 x 128 planes on a 256 x 256 fine grid, without
+        // a batch sum
+        if constexpr (std::is_same_v<V, float2>) {
+            if (!sum_batch && nufft_fused_enabled() && !fft_force_stockham() &&
+                nufft_inv_xy_supported(ny, nx, Ly, Lx, g.shift_out[1], g.shift_out[2]) &&
+                (!deapod || (deapod[0] && deapod[1] && deapod[2]))) {
+                NufftInvXyArgs f{};
+                f.in = bufB; f.in_b = nz * Ly * Lx;
+                f.out = img; f.out_b = nz * ny * nx;
+                f.scale = scale;
+                f.m1 = m1; f.m2 = m2; f.io = io;
+                f.dz = deapod ? deapod[0] : nullptr; f.dy = deapod ? deapod[1] : nullptr;
+                f.dx = deapod ? deapod[2] : nullptr;
+                f.accumulate = accumulate;
+                f.nbatch = int(B); f.nz = int(nz); f.sign = sign;
+                launch_nufft_inv_xy(f, s);
+                return;
+            }
+        }
     }
     ColArgs<V> c = make_col_args<V>(int(Ly));
     c.in = ysrc; c.ni = int(Lx); c.no = int(nz); c.nbatch = int(B);

@@ -56,6 +56,24 @@ struct NufftFwdXyArgs {
 bool nufft_fwd_xy_supported(int64_t ny, int64_t nx, int64_t Ly, int64_t Lx, int shift_y, int shift_x);
 void launch_nufft_fwd_xy(const NufftFwdXyArgs& a, hipStream_t s);
 
+// NUFFT type-1 fine grid: the y and x inverse passes with their crops of the
+// same plans, fused (nufft_fused.hip); grid [B, nz, 256, 256] (after the z
+// pass) -> img [B, nz, 128, 128] with the row pass's epilogue: scale, the
+// store diagonals, deapodisation, accumulate (no batch sum)
+struct NufftInvXyArgs {
+    const float2* in;
+    int64_t in_b;
+    float2* out;
+    int64_t out_b;
+    float scale;
+    DiagOp<float2> m1, m2, io;                     // store diagonals, as the row pass takes them
+    const float* dz; const float* dy; const float* dx;   // deapodisation (all or none)
+    int accumulate;
+    int nbatch, nz, sign, groups;
+};
+bool nufft_inv_xy_supported(int64_t ny, int64_t nx, int64_t Ly, int64_t Lx, int shift_y, int shift_x);
+void launch_nufft_inv_xy(const NufftInvXyArgs& a, hipStream_t s);
+
 // plane shapes the fused passes are built for (ny == nx, 64, 96 or 128)
 bool toep_crop_sum_supported(int64_t ny, int64_t nx);
 // groups: coil groups per plane (partial sums added by a second kernel)

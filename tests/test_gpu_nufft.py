@@ -358,6 +358,31 @@ def test_fused_fine_grid_sense_forward_128_planes():
     report("fused fine-grid SENSE forward (3, 128, 128) C=3", rel_l2(y, sense_forward(x, S, coords)), TOL)
 
 
+@pytest.mark.parametrize("combine", [False, True])
+def test_fused_fine_grid_sense_adjoint_128_planes(combine):
+    """Time-segmented SENSE adjoint (L = 2) on 128 x 128 planes.  Per coil,
+    the type-1 y and x passes run as one fused kernel whose epilogue applies
+    conj(S_c) and conj(cjl_l) as store diagonals and accumulates over the
+    segments; coil-combined, the batch sum keeps the separate passes.  Both
+    vs the FP64 NDFT model."""
+    from hastycucompute_amd import mri
+    from oracle.normal_ops import sense_adjoint_offres
+    shape, C, L, M = (3, 128, 128), 3, 2, 1200
+    rng = np.random.default_rng(6)
+    coords = rand_coords(rng, 3, M)
+    S = crandn(rng, (C,) + shape)
+    y = crandn(rng, (C, M))
+    bil = crandn(rng, (L, M))
+    cjl = crandn(rng, (L,) + shape)
+    print(f"[case] fused fine-grid SENSE adjoint {shape} C={C} L={L} combine={combine}", flush=True)
+    op = mri.SenseOffresonance([cuda(k) for k in coords], cuda(S), cuda(bil), cuda(cjl))
+    got = host(op.adjoint(cuda(y), combine=combine))
+    ref = sense_adjoint_offres(y, S, coords, bil, cjl)
+    if combine:
+        ref = ref.sum(0)
+    report(f"fused fine-grid SENSE adjoint {shape} C={C} L={L} combine={combine}", rel_l2(got, ref), TOL)
+
+
 @pytest.mark.parametrize("tol,dtype", [(1e-3, torch.float32), (1e-4, torch.float32), (1e-5, torch.float64)])
 @pytest.mark.parametrize("shape,M,ntransf", [((24, 20, 16), 3000, 1), ((20, 16, 24), 2500, 3),
                                              ((48, 40), 3000, 2)])
