@@ -323,12 +323,12 @@ def test_setpts_is_deterministic():
     print("[parity] setpts: bitwise identical type 1 / type 2 over repeated setpts and plans", flush=True)
 
 
-@pytest.mark.parametrize("sign,ntransf", [(-1, 2), (1, 3)])
+@pytest.mark.parametrize("sign,ntransf", [(-1, 2), (1, 3), (1, 1)])
 def test_fused_fine_grid_xy_pass_128_planes(sign, ntransf):
-    """Type 2 with 128 x 128 planes on a 256 x 256 fine grid runs its x and y
+    """With 128 x 128 planes on a 256 x 256 fine grid, type 2 runs its x and y
     passes as one kernel (y split by residue classes, x in natural order, the
-    centred-mode shift folded in): vs FP64 NDFT, both signs, batched; the
-    type-1 round trip of the same plan pair is unchanged."""
+    centred-mode shift folded in) and type 1 its y and x inverse passes: vs
+    FP64 NDFT, both signs, one and several transforms."""
     shape = (4, 128, 128)
     rng = np.random.default_rng(21 + ntransf)
     M = 1500
