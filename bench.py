@@ -245,16 +245,19 @@ def reduce_pmc(fetch_dir, write_dir, workload):
 
 _TRACE_CMD = ("cd /tmp && export TMPDIR=/tmp && cd $REPO && rocprofv3 --kernel-trace --stats --output-format csv "
               "-d {d} -o run -- python -u bench.py --steps 3 --warmup 1 --no-cpu-baseline")
-# the launches this script times after its calibration kernel: (apply, type 2,
-# type 1) three times, each as (label, kernel-name substring)
+# per-kernel HIP-event timing: launches of each operator, one operator after
+# another (consecutive launches of one operator, as in the timed phases)
+PROFILE_LAUNCHES = 10
+# the launches this script times after its calibration kernel: the apply, then
+# type 2, then type 1, PROFILE_LAUNCHES times each, as (label, kernel-name substring)
 _TIMED_SEQ = ([("toep_F12_xy_pad_fft", "toep_pad_fft_kernel"),
                ("toep_P3_z_psf", "<c64, 16, 16, -1, 1, 3>"), ("toep_F45_yx_ifft_crop_sum", "toep_crop_sum_kernel"),
-               ("toep_F45 sum_partials", "sum_partials")] +
+               ("toep_F45 sum_partials", "sum_partials")] * PROFILE_LAUNCHES +
               [("fwd_xy_pad_fft", "nufft_fwd_xy_kernel"),
-               ("fwd_z_pad_fft", "<c64, 16, 16, -1, 0, 1>"), ("nufft_interp", "interp_pair_kernel<6,")] +
+               ("fwd_z_pad_fft", "<c64, 16, 16, -1, 0, 1>"), ("nufft_interp", "interp_pair_kernel<6,")] * PROFILE_LAUNCHES +
               [("nufft_sort_values", "sort_values_kernel<c64"), ("nufft_spread", "spread_lanes_kernel<6, 32>"),
                ("nufft_spread_reduce", "reduce_split_tiles"), ("inv_z_fft_crop", "<c64, 16, 16, 1, 0, 4>"),
-               ("inv_xy_fft_crop", "nufft_inv_xy_kernel")]) * 3
+               ("inv_xy_fft_crop", "nufft_inv_xy_kernel")] * PROFILE_LAUNCHES)
 
 
 def _short_kernel(name):
@@ -306,7 +309,7 @@ def summarize_trace(trace_dir, workload, top=30):
         timed.setdefault((tag, name), []).append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6)
     out += ["", "## Launches timed by bench.py's roofline", "",
             "The launches that follow the calibration kernel (`torch.neg` over 1 GiB): an A^H A apply (the fused x/y "
-            "pad-FFT pass, P3, the fused y/x inverse pass with the coil sum, and the coil-group partials), a type-2 and a type-1 transform at full size, three times over, which "
+            f"pad-FFT pass, P3, the fused y/x inverse pass with the coil sum, and the coil-group partials), a type-2 and a type-1 transform at full size, {PROFILE_LAUNCHES} times each, one operator after another, which "
             "bench.py times with HIP events on the library stream. Several passes share a kernel instance, so "
             "each launch is labelled by its place in that sequence.", "",
             "| pass | kernel | launches | avg ms |", "|---|---|---:|---:|"]
@@ -667,11 +670,14 @@ def main():
     torch.cuda.synchronize()
     del calib, calib_dst
 
-    # per-kernel HIP-event timing (library stream), 3 launches of each op
+    # per-kernel HIP-event timing (library stream), PROFILE_LAUNCHES
+    # consecutive launches of each operator
     _lib.profile_enable(True)
-    for _ in range(3):
+    for _ in range(PROFILE_LAUNCHES):
         op.apply(x, out)
+    for _ in range(PROFILE_LAUNCHES):
         p2.execute(img_c, y)
+    for _ in range(PROFILE_LAUNCHES):
         p1.execute(y, back)
     kern = _lib.profile_read()
     _lib.profile_enable(False)
